@@ -1,0 +1,459 @@
+"""ctypes binding of libtqsv (include/tqsv.h).
+
+This module is the only place that touches the C ABI.  It loads the in-tree
+``libtqsv.so`` and fails loudly if it is missing: the statevector engine has
+no CPU fallback.  States are torch complex64 tensors on a HIP device; their
+data pointers and the current HIP stream are handed to the library.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+from typing import Optional, Sequence
+
+import numpy as np
+
+_LIB_PATH = Path(__file__).resolve().parent / "libtqsv.so"
+_lib: Optional[ctypes.CDLL] = None
+
+OP_MAT1 = 1
+OP_MAT2 = 2
+OP_PROT = 3
+
+# numpy mirror of struct tqsv_op (include/tqsv.h)
+OP_DTYPE = np.dtype(
+    [
+        ("kind", np.int32),
+        ("t", np.int32, (2,)),
+        ("param", np.int32),
+        ("cmask", np.uint64),
+        ("cval", np.uint64),
+        ("xmask", np.uint64),
+        ("zmask", np.uint64),
+        ("theta", np.float64),
+        ("gx", np.uint64),
+        ("gz", np.uint64),
+        ("gcoef", np.float64),
+        ("m", np.float64, (32,)),
+    ],
+    align=True,
+)
+assert OP_DTYPE.itemsize == 336
+
+
+class PlanOpts(ctypes.Structure):
+    _fields_ = [
+        ("tile_bits", ctypes.c_int32),
+        ("min_low_bits", ctypes.c_int32),
+        ("max_cost", ctypes.c_int32),
+        ("fuse", ctypes.c_int32),
+    ]
+
+
+class PlanInfo(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_int32),
+        ("ops_in", ctypes.c_int32),
+        ("gates_after_merge", ctypes.c_int32),
+        ("passes", ctypes.c_int32),
+        ("phases", ctypes.c_int32),
+        ("tile_bits", ctypes.c_int32),
+        ("bytes_per_run", ctypes.c_int64),
+        ("program_bytes", ctypes.c_int64),
+    ]
+
+
+class HamInfo(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_int32),
+        ("nterms", ctypes.c_int32),
+        ("passes", ctypes.c_int32),
+        ("phases", ctypes.c_int32),
+        ("flip_groups", ctypes.c_int32),
+        ("unfused_terms", ctypes.c_int32),
+    ]
+
+
+class NativeError(RuntimeError):
+    """A libtqsv call returned an error code."""
+
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_U64 = ctypes.c_uint64
+_D = ctypes.c_double
+_DP = ctypes.POINTER(ctypes.c_double)
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "tqsv_version": (_I, []),
+    "tqsv_last_error": (ctypes.c_char_p, []),
+    "tqsv_init_basis": (_I, [_P, _I, _U64, _P]),
+    "tqsv_apply_1q": (_I, [_P, _I, _I, _DP, _P]),
+    "tqsv_apply_2q": (_I, [_P, _I, _I, _I, _DP, _P]),
+    "tqsv_apply_ctrl": (_I, [_P, _I, _U64, _U64, _I, _DP, _P]),
+    "tqsv_apply_diag": (_I, [_P, _I, _I, ctypes.POINTER(ctypes.c_int), _DP, _P]),
+    "tqsv_pauli_rot": (_I, [_P, _I, _U64, _U64, _D, _P]),
+    "tqsv_project_z": (_I, [_P, _I, _I, _I, _DP, _P]),
+    "tqsv_scale": (_I, [_P, _I, _D, _D, _P]),
+    "tqsv_norm2": (_I, [_P, _I, _DP, _P]),
+    "tqsv_expect_z": (_I, [_P, _I, _I, _DP, _P]),
+    "tqsv_dot": (_I, [_P, _P, _I, _U64, _U64, _DP, _P]),
+    "tqsv_probs": (_I, [_P, _I, _P, _P]),
+    "tqsv_plan_create": (_I, [_I, _P, _I, ctypes.POINTER(PlanOpts), ctypes.POINTER(_P)]),
+    "tqsv_plan_update": (_I, [_P, _P, _I, _P]),
+    "tqsv_plan_get_info": (_I, [_P, ctypes.POINTER(PlanInfo)]),
+    "tqsv_plan_inspect": (_I, [_I, _P, _I, ctypes.POINTER(PlanOpts), ctypes.POINTER(PlanInfo)]),
+    "tqsv_plan_run": (_I, [_P, _P, _P]),
+    "tqsv_plan_destroy": (None, [_P]),
+    "tqsv_apply_fused": (_I, [_P, _I, _P, _I, ctypes.POINTER(PlanOpts), _P]),
+    "tqsv_ham_create": (_I, [_I, _I, _P, _P, _P, ctypes.POINTER(_P)]),
+    "tqsv_ham_expval": (_I, [_P, _P, _DP, _DP, _DP, _P]),
+    "tqsv_ham_apply": (_I, [_P, _P, _P, _P]),
+    "tqsv_ham_get_info": (_I, [_P, ctypes.POINTER(HamInfo)]),
+    "tqsv_ham_destroy": (None, [_P]),
+    "tqsv_expval": (_I, [_P, _I, _I, _P, _P, _P, _DP, _DP, _DP, _P]),
+    "tqsv_adjoint": (_I, [_P, _P, _P, _P, _I, _DP, _DP, _P]),
+    "tqsv_vjp": (_I, [_P, _P, _P, _I, _DP, _P]),
+}
+
+
+def exported_symbols() -> list[str]:
+    return list(_SIGS)
+
+
+def lib_path() -> Path:
+    return _LIB_PATH
+
+
+def load() -> ctypes.CDLL:
+    """Load libtqsv.so (built in-tree by ``tyxonq_amd._build``)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not _LIB_PATH.exists():
+        raise NativeError(
+            f"{_LIB_PATH} is missing: build it with `python -m tyxonq_amd._build` "
+            "(the statevector engine has no CPU fallback)"
+        )
+    lib = ctypes.CDLL(str(_LIB_PATH))
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        msg = load().tqsv_last_error().decode(errors="replace")
+        if rc == -1:
+            raise ValueError(msg)
+        if rc == -4:
+            raise NotImplementedError(msg)
+        raise NativeError(f"libtqsv error {rc}: {msg}")
+
+
+# ---- torch plumbing ---------------------------------------------------------
+
+def _torch():
+    import torch
+
+    return torch
+
+
+def device() -> "object":
+    torch = _torch()
+    if not torch.cuda.is_available():
+        raise NativeError("no HIP device is visible: the tyxonq_amd statevector engine needs a GPU")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def stream_ptr() -> int:
+    torch = _torch()
+    return int(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t) -> int:
+    torch = _torch()
+    if not isinstance(t, torch.Tensor) or t.dtype != torch.complex64 or not t.is_cuda:
+        raise TypeError("expected a complex64 tensor on the GPU")
+    if not t.is_contiguous():
+        raise ValueError("state tensor must be contiguous")
+    return int(t.data_ptr())
+
+
+def _nqubits(t) -> int:
+    d = int(t.numel())
+    n = d.bit_length() - 1
+    if d < 2 or (1 << n) != d:
+        raise ValueError(f"state length {d} is not a power of two >= 2")
+    return n
+
+
+def _dbuf(values: Sequence[float]):
+    arr = (ctypes.c_double * len(values))(*[float(v) for v in values])
+    return arr
+
+
+def new_state(n: int, index: int = 0):
+    torch = _torch()
+    psi = torch.empty(1 << n, dtype=torch.complex64, device=device())
+    init_basis(psi, index)
+    return psi
+
+
+def init_basis(psi, index: int = 0) -> None:
+    _check(load().tqsv_init_basis(_ptr(psi), _nqubits(psi), int(index), stream_ptr()))
+
+
+def mat_buffer(m) -> "ctypes.Array":
+    m = np.asarray(m, dtype=np.complex128).reshape(-1)
+    flat = np.empty(2 * m.size)
+    flat[0::2] = m.real
+    flat[1::2] = m.imag
+    return _dbuf(flat)
+
+
+def apply_1q(psi, bit: int, U) -> None:
+    _check(load().tqsv_apply_1q(_ptr(psi), _nqubits(psi), int(bit), mat_buffer(U), stream_ptr()))
+
+
+def apply_2q(psi, bit_hi: int, bit_lo: int, U) -> None:
+    _check(
+        load().tqsv_apply_2q(
+            _ptr(psi), _nqubits(psi), int(bit_hi), int(bit_lo), mat_buffer(U), stream_ptr()
+        )
+    )
+
+
+def apply_ctrl(psi, ctrl_mask: int, ctrl_val: int, bit: int, U) -> None:
+    _check(
+        load().tqsv_apply_ctrl(
+            _ptr(psi), _nqubits(psi), int(ctrl_mask), int(ctrl_val), int(bit), mat_buffer(U),
+            stream_ptr(),
+        )
+    )
+
+
+def apply_diag(psi, bits: Sequence[int], phases) -> None:
+    bits = [int(b) for b in bits]
+    arr = (ctypes.c_int * len(bits))(*bits)
+    _check(
+        load().tqsv_apply_diag(
+            _ptr(psi), _nqubits(psi), len(bits), arr, mat_buffer(phases), stream_ptr()
+        )
+    )
+
+
+def pauli_rot(psi, xmask: int, zmask: int, theta: float) -> None:
+    _check(
+        load().tqsv_pauli_rot(
+            _ptr(psi), _nqubits(psi), int(xmask), int(zmask), float(theta), stream_ptr()
+        )
+    )
+
+
+def project_z(psi, bit: int, keep: int) -> float:
+    out = ctypes.c_double(0.0)
+    _check(
+        load().tqsv_project_z(
+            _ptr(psi), _nqubits(psi), int(bit), int(keep), ctypes.byref(out), stream_ptr()
+        )
+    )
+    return out.value
+
+
+def scale(psi, value: complex) -> None:
+    v = complex(value)
+    _check(load().tqsv_scale(_ptr(psi), _nqubits(psi), v.real, v.imag, stream_ptr()))
+
+
+def norm2(psi) -> float:
+    out = ctypes.c_double(0.0)
+    _check(load().tqsv_norm2(_ptr(psi), _nqubits(psi), ctypes.byref(out), stream_ptr()))
+    return out.value
+
+
+def expect_z(psi, bit: int) -> float:
+    out = ctypes.c_double(0.0)
+    _check(
+        load().tqsv_expect_z(_ptr(psi), _nqubits(psi), int(bit), ctypes.byref(out), stream_ptr())
+    )
+    return out.value
+
+
+def dot(u, v, xmask: int = 0, zmask: int = 0) -> complex:
+    """sum_i (-1)^{|i&z|} conj(u[i]) v[i^x], accumulated in FP64."""
+    out = (ctypes.c_double * 2)()
+    n = _nqubits(u)
+    if _nqubits(v) != n:
+        raise ValueError("state sizes differ")
+    _check(load().tqsv_dot(_ptr(u), _ptr(v), n, int(xmask), int(zmask), out, stream_ptr()))
+    return complex(out[0], out[1])
+
+
+def probs(psi):
+    torch = _torch()
+    out = torch.empty(psi.numel(), dtype=torch.float64, device=psi.device)
+    _check(load().tqsv_probs(_ptr(psi), _nqubits(psi), int(out.data_ptr()), stream_ptr()))
+    return out
+
+
+# ---- plans -----------------------------------------------------------------
+
+def _opts(tile_bits: int = 0, min_low_bits: int = 0, max_cost: int = 0, fuse: bool = True):
+    return PlanOpts(int(tile_bits), int(min_low_bits), int(max_cost), 1 if fuse else 0)
+
+
+def inspect(n: int, ops: np.ndarray, *, tile_bits: int = 0, min_low_bits: int = 0,
+            max_cost: int = 0, fuse: bool = True) -> dict:
+    """Plan on the host only and return the plan statistics (no GPU needed)."""
+    ops = np.ascontiguousarray(ops, dtype=OP_DTYPE)
+    o = _opts(tile_bits, min_low_bits, max_cost, fuse)
+    inf = PlanInfo()
+    _check(load().tqsv_plan_inspect(int(n), ops.ctypes.data if len(ops) else None, len(ops),
+                                    ctypes.byref(o), ctypes.byref(inf)))
+    return {f: getattr(inf, f) for f, _ in PlanInfo._fields_}
+
+
+class Plan:
+    """A planned (fused) circuit: owns its device program."""
+
+    def __init__(self, n: int, ops: np.ndarray, *, tile_bits: int = 0, min_low_bits: int = 0,
+                 max_cost: int = 0, fuse: bool = True):
+        load()
+        self.n = int(n)
+        self._ops = np.ascontiguousarray(ops, dtype=OP_DTYPE)
+        self._h = ctypes.c_void_p()
+        o = _opts(tile_bits, min_low_bits, max_cost, fuse)
+        _check(
+            load().tqsv_plan_create(
+                self.n, self._ops.ctypes.data if len(self._ops) else None, len(self._ops),
+                ctypes.byref(o), ctypes.byref(self._h),
+            )
+        )
+        self.nparams = int(self._ops["param"].max()) + 1 if len(self._ops) else 0
+
+    def update(self, ops: np.ndarray) -> None:
+        self._ops = np.ascontiguousarray(ops, dtype=OP_DTYPE)
+        _check(
+            load().tqsv_plan_update(
+                self._h, self._ops.ctypes.data if len(self._ops) else None, len(self._ops),
+                stream_ptr(),
+            )
+        )
+
+    def info(self) -> dict:
+        inf = PlanInfo()
+        _check(load().tqsv_plan_get_info(self._h, ctypes.byref(inf)))
+        return {f: getattr(inf, f) for f, _ in PlanInfo._fields_}
+
+    def run(self, psi) -> None:
+        if _nqubits(psi) != self.n:
+            raise ValueError(f"plan is for {self.n} qubits, state has {_nqubits(psi)}")
+        _check(load().tqsv_plan_run(self._h, _ptr(psi), stream_ptr()))
+
+    def adjoint(self, ham: "Ham", psi, lam, nparams: Optional[int] = None):
+        """Forward-run psi, then the adjoint sweep.  Returns (energy, grad[nparams])."""
+        npar = self.nparams if nparams is None else int(nparams)
+        grad = np.zeros(max(npar, 1))
+        e = ctypes.c_double(0.0)
+        _check(
+            load().tqsv_adjoint(
+                self._h, ham._h, _ptr(psi), _ptr(lam), npar,
+                grad.ctypes.data_as(_DP), ctypes.byref(e), stream_ptr(),
+            )
+        )
+        return e.value, grad[:npar]
+
+    def vjp(self, psi, lam, nparams: Optional[int] = None) -> np.ndarray:
+        npar = self.nparams if nparams is None else int(nparams)
+        grad = np.zeros(max(npar, 1))
+        _check(
+            load().tqsv_vjp(self._h, _ptr(psi), _ptr(lam), npar, grad.ctypes.data_as(_DP),
+                            stream_ptr())
+        )
+        return grad[:npar]
+
+    def close(self) -> None:
+        if self._h:
+            load().tqsv_plan_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Ham:
+    """A Pauli sum planned for n-qubit expectations (owns device buffers)."""
+
+    def __init__(self, n: int, xmask: Sequence[int], zmask: Sequence[int], coeff: Sequence[float]):
+        load()
+        self.n = int(n)
+        self.x = np.ascontiguousarray(np.asarray(xmask, dtype=np.uint64))
+        self.z = np.ascontiguousarray(np.asarray(zmask, dtype=np.uint64))
+        self.c = np.ascontiguousarray(np.asarray(coeff, dtype=np.float64))
+        if not (len(self.x) == len(self.z) == len(self.c)):
+            raise ValueError("xmask, zmask and coeff must have equal length")
+        self._h = ctypes.c_void_p()
+        nt = len(self.c)
+        _check(
+            load().tqsv_ham_create(
+                self.n, nt, self.x.ctypes.data if nt else None, self.z.ctypes.data if nt else None,
+                self.c.ctypes.data if nt else None, ctypes.byref(self._h),
+            )
+        )
+
+    @property
+    def nterms(self) -> int:
+        return len(self.c)
+
+    def expval(self, psi, per_term: bool = False):
+        """Returns (energy, norm2) or (energy, norm2, terms)."""
+        if _nqubits(psi) != self.n:
+            raise ValueError(f"Hamiltonian is for {self.n} qubits, state has {_nqubits(psi)}")
+        terms = np.zeros(max(self.nterms, 1))
+        e = ctypes.c_double(0.0)
+        nrm = ctypes.c_double(0.0)
+        _check(
+            load().tqsv_ham_expval(
+                self._h, _ptr(psi), terms.ctypes.data_as(_DP), ctypes.byref(e),
+                ctypes.byref(nrm), stream_ptr(),
+            )
+        )
+        if per_term:
+            return e.value, nrm.value, terms[: self.nterms]
+        return e.value, nrm.value
+
+    def apply(self, psi, out) -> None:
+        _check(load().tqsv_ham_apply(self._h, _ptr(psi), _ptr(out), stream_ptr()))
+
+    def info(self) -> dict:
+        inf = HamInfo()
+        _check(load().tqsv_ham_get_info(self._h, ctypes.byref(inf)))
+        return {f: getattr(inf, f) for f, _ in HamInfo._fields_}
+
+    def close(self) -> None:
+        if self._h:
+            load().tqsv_ham_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def version() -> int:
+    return int(load().tqsv_version())
+
+
+if os.environ.get("TQSV_EAGER_LOAD"):
+    load()

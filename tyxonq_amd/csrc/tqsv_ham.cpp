@@ -1,0 +1,617 @@
+// tqsv_ham.cpp - Pauli-sum expectation (fused tile reductions), H|psi>, and
+// the adjoint / vector-Jacobian sweeps.
+//
+// Expectation.  Terms are grouped by flip mask x.  A group is evaluated in a
+// register phase whose register positions contain x: every thread then holds
+// both members of each pair (i, i^x) and forms conj(psi[i^x]) psi[i] once per
+// pair; each term of the group only adds a signed copy of it (the sign of Z
+// parts splits into a per-slot constant and a per-thread parity).  Groups are
+// packed into read-only tile passes, so one pass serves every group whose
+// flip bits lie in its tile.  Sums go through per-wave FP64 accumulators in
+// LDS, per-work-group partials and a fixed-order final sum: the result does
+// not depend on launch timing.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+
+#include "tqsv_planner.h"
+
+using namespace tqsv;
+
+struct tqsv_ham {
+  int n = 0;
+  int nterms = 0;
+  std::vector<uint64_t> x, z;
+  std::vector<double> coeff;
+  // expectation plan
+  std::vector<PassPlan> passes;
+  std::vector<uint32_t> prog;
+  uint32_t* dprog = nullptr;
+  // raw output slot per term (index nterms is the norm); mode 0: one tile sum,
+  // mode 1: (re, im) pair from the dot kernels
+  std::vector<int> tslot, tmode;
+  uint32_t nout = 0;
+  std::vector<int> small_terms;   // evaluated by the one-block-per-term kernel
+  uint32_t small_base = 0;
+  std::vector<int> big_terms;     // evaluated by a grid-wide dot kernel each
+  uint64_t* d_sx = nullptr;
+  uint64_t* d_sz = nullptr;
+  double* dpartials = nullptr;
+  size_t partials_cap = 0;
+  double* dout = nullptr;
+  std::vector<double> hout;
+  // H|psi>
+  uint32_t ngroups = 0;
+  uint64_t* d_gx = nullptr;
+  uint32_t* d_gstart = nullptr;
+  uint64_t* d_tz = nullptr;
+  float2* d_tc = nullptr;
+
+  ~tqsv_ham() {
+    (void)hipFree(dprog);
+    (void)hipFree(d_sx);
+    (void)hipFree(d_sz);
+    (void)hipFree(dpartials);
+    (void)hipFree(dout);
+    (void)hipFree(d_gx);
+    (void)hipFree(d_gstart);
+    (void)hipFree(d_tz);
+    (void)hipFree(d_tc);
+  }
+};
+
+namespace {
+
+inline int popc(uint64_t v) { return __builtin_popcountll(v); }
+inline int ctz(uint64_t v) { return __builtin_ctzll(v); }
+inline uint64_t lowmask(int L) { return L >= 64 ? ~0ull : ((1ull << L) - 1ull); }
+
+template <typename T>
+int upload(T** dptr, const std::vector<T>& v) {
+  if (v.empty()) return 0;
+  TQSV_HIP(hipMalloc(reinterpret_cast<void**>(dptr), v.size() * sizeof(T)));
+  TQSV_HIP(hipMemcpy(*dptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  return 0;
+}
+
+constexpr uint32_t NRED_MAX = 2048;
+
+int plan_expval(tqsv_ham* h) {
+  const int n = h->n;
+  const int T = effective_tile_bits(n, 13, false);
+  const bool tiles = T >= 11;
+  const int nt = h->nterms + 1;  // + norm
+  auto tx = [&](int t) { return t < h->nterms ? h->x[t] : 0ull; };
+  auto tz = [&](int t) { return t < h->nterms ? h->z[t] : 0ull; };
+  h->tslot.assign(nt, -1);
+  h->tmode.assign(nt, 0);
+
+  std::map<uint64_t, std::vector<int>> groups;
+  for (int t = 0; t < nt; ++t) {
+    if (!tiles) {
+      h->small_terms.push_back(t);
+    } else if (popc(tx(t)) > RB) {
+      h->big_terms.push_back(t);
+    } else {
+      groups[tx(t)].push_back(t);
+    }
+  }
+  uint32_t nout = 0;
+  if (tiles) {
+    const int TB = T - RB;
+    const int Lmin = std::min(5, T - RB);
+    // most constrained groups first
+    std::vector<uint64_t> order;
+    for (auto& kv : groups) order.push_back(kv.first);
+    std::stable_sort(order.begin(), order.end(), [&](uint64_t a, uint64_t b) {
+      return popc(a & ~lowmask(Lmin)) > popc(b & ~lowmask(Lmin));
+    });
+    std::vector<char> done(order.size(), 0);
+    size_t left = order.size();
+    while (left) {
+      // tile bit set
+      uint64_t S = lowmask(Lmin);
+      uint32_t nred = 0;
+      for (size_t i = 0; i < order.size(); ++i) {
+        if (done[i]) continue;
+        const uint64_t need = order[i] & ~S;
+        const uint32_t c = (uint32_t)groups[order[i]].size();
+        if (popc(S | need) > T || nred + c > NRED_MAX) continue;
+        S |= need;
+        nred += c;
+      }
+      int L = Lmin;
+      for (;;) {
+        const int nL = T - popc(S & ~lowmask(L));
+        if (nL <= L) break;
+        L = nL;
+      }
+      std::vector<int> H;
+      for (uint64_t v = S & ~lowmask(L); v; v &= v - 1) H.push_back(ctz(v));
+      const uint64_t tileb = lowmask(L) | (S & ~lowmask(L));
+      auto pos = [&](int b) {
+        if (b < L) return b;
+        for (size_t k = 0; k < H.size(); ++k)
+          if (H[k] == b) return L + (int)k;
+        return -1;
+      };
+      auto posmask = [&](uint64_t bits) {
+        uint32_t m = 0;
+        for (uint64_t v = bits; v; v &= v - 1) m |= 1u << pos(ctz(v));
+        return m;
+      };
+      // groups of this pass
+      std::vector<size_t> mine;
+      nred = 0;
+      for (size_t i = 0; i < order.size(); ++i) {
+        if (done[i] || (order[i] & ~tileb)) continue;
+        const uint32_t c = (uint32_t)groups[order[i]].size();
+        if (nred + c > NRED_MAX) continue;
+        mine.push_back(i);
+        nred += c;
+        done[i] = 1;
+        --left;
+      }
+      // phases: pack flip masks into register sets
+      PassPlan pp;
+      pp.tile = true;
+      pp.tb = TB;
+      pp.L = (uint32_t)L;
+      pp.m = (uint32_t)H.size();
+      for (size_t k = 0; k < H.size(); ++k) pp.H[k] = (uint32_t)H[k];
+      pp.prog_off = h->prog.size();
+      pp.red_base = nout;
+      std::vector<uint32_t>& w = h->prog;
+      for (int k = 0; k < PH_SIZE; ++k) w.push_back(0);
+      std::vector<size_t> prem = mine;
+      uint32_t local = 0;
+      uint32_t nphase = 0;
+      while (!prem.empty()) {
+        uint32_t R = 0;
+        std::vector<size_t> sel, keep;
+        for (size_t i : prem) {
+          const uint32_t need = posmask(order[i]);
+          if (popc((uint64_t)(R | need)) <= RB) R |= need;
+        }
+        for (int p = T - 1; popc((uint64_t)R) < RB && p >= 0; --p) R |= 1u << p;
+        for (size_t i : prem) {
+          if (posmask(order[i]) & ~R)
+            keep.push_back(i);
+          else
+            sel.push_back(i);
+        }
+        std::array<uint32_t, RB> Rp{};
+        int j = 0;
+        for (int p = 0; p < T; ++p)
+          if ((R >> p) & 1u) Rp[j++] = (uint32_t)p;
+        auto slot_of_bit = [&](int b) -> int {
+          const int p = pos(b);
+          for (int q = 0; q < RB; ++q)
+            if ((int)Rp[q] == p) return q;
+          return -1;
+        };
+        auto slotmask = [&](uint64_t bits) {
+          uint32_t m = 0;
+          for (uint64_t v = bits; v; v &= v - 1) {
+            const int s = slot_of_bit(ctz(v));
+            if (s >= 0) m |= 1u << s;
+          }
+          return m;
+        };
+        const size_t hdr = w.size();
+        for (int k = 0; k < PS_SIZE; ++k) w.push_back(0);
+        w[hdr + PS_NSTEPS] = (uint32_t)sel.size();
+        for (int r = 0; r < RB; ++r) w[hdr + PS_R + r] = Rp[r];
+        for (size_t i : sel) {
+          const uint64_t xm = order[i];
+          const std::vector<int>& terms = groups[xm];
+          const size_t at = w.size();
+          w.push_back(OP_RED);
+          w.push_back(slotmask(xm));
+          w.push_back(xm ? RED_PAIR : RED_DIAG);
+          w.push_back((uint32_t)terms.size());
+          for (int t : terms) {
+            const uint64_t zz = tz(t);
+            const uint32_t zs = slotmask(zz);
+            uint32_t sm = 0;
+            for (int s = 0; s < NSLOT; ++s)
+              if (popc((uint64_t)(s & zs)) & 1) sm |= 1u << s;
+            const int k = popc(xm & zz) & 3;
+            w.push_back(local);
+            w.push_back((uint32_t)zz);
+            w.push_back((uint32_t)(zz >> 32));
+            w.push_back(sm);
+            w.push_back(xm ? (uint32_t)(k & 1) : 0u);
+            h->tslot[t] = (int)(nout + local);
+            h->tmode[t] = 0;
+            ++local;
+          }
+          w[at] |= (uint32_t)((w.size() - at) << 16);
+        }
+        ++nphase;
+        prem.swap(keep);
+      }
+      uint32_t* hd = w.data() + pp.prog_off;
+      hd[PH_FLAGS] = 0;
+      hd[PH_L] = pp.L;
+      hd[PH_M] = pp.m;
+      hd[PH_NPHASE] = nphase;
+      hd[PH_NRED] = local;
+      hd[PH_WORDS] = (uint32_t)(w.size() - pp.prog_off);
+      for (uint32_t k = 0; k < pp.m; ++k) hd[PH_H + k] = pp.H[k];
+      pp.prog_words = w.size() - pp.prog_off;
+      pp.nred = local;
+      pp.ntiles = (uint32_t)(1ull << (n - T));
+      pp.grid = std::min<uint32_t>(pp.ntiles, 2048u);
+      pp.lds = pass_lds_bytes(TB, false, local);
+      h->partials_cap = std::max(h->partials_cap, (size_t)pp.grid * local);
+      nout += local;
+      h->passes.push_back(pp);
+    }
+  }
+  // dot-kernel terms: (re, im) pairs
+  h->small_base = nout;
+  for (int t : h->small_terms) {
+    h->tslot[t] = (int)nout;
+    h->tmode[t] = 1;
+    nout += 2;
+  }
+  for (int t : h->big_terms) {
+    h->tslot[t] = (int)nout;
+    h->tmode[t] = 1;
+    nout += 2;
+  }
+  h->partials_cap = std::max(h->partials_cap, (size_t)reduce_grid(n) * 2);
+  h->nout = nout;
+  h->hout.assign(nout, 0.0);
+  if (upload(&h->dprog, h->prog)) return TQSV_EDEVICE;
+  std::vector<uint64_t> sx, sz;
+  for (int t : h->small_terms) {
+    sx.push_back(tx(t));
+    sz.push_back(tz(t));
+  }
+  if (upload(&h->d_sx, sx)) return TQSV_EDEVICE;
+  if (upload(&h->d_sz, sz)) return TQSV_EDEVICE;
+  TQSV_HIP(hipMalloc(reinterpret_cast<void**>(&h->dpartials), h->partials_cap * sizeof(double)));
+  TQSV_HIP(hipMalloc(reinterpret_cast<void**>(&h->dout), std::max<uint32_t>(nout, 1) * sizeof(double)));
+  return 0;
+}
+
+int plan_apply(tqsv_ham* h) {
+  std::map<uint64_t, std::vector<int>> groups;
+  for (int t = 0; t < h->nterms; ++t) groups[h->x[t]].push_back(t);
+  std::vector<uint64_t> gx, tzv;
+  std::vector<uint32_t> gstart;
+  std::vector<float2> tc;
+  for (auto& kv : groups) {
+    gx.push_back(kv.first);
+    gstart.push_back((uint32_t)tzv.size());
+    for (int t : kv.second) {
+      tzv.push_back(h->z[t]);
+      const int k = popc(h->x[t] & h->z[t]) & 3;
+      // coeff * i^k
+      const double c = h->coeff[t];
+      float2 v;
+      switch (k) {
+        case 0: v = make_float2((float)c, 0.f); break;
+        case 1: v = make_float2(0.f, (float)c); break;
+        case 2: v = make_float2((float)-c, 0.f); break;
+        default: v = make_float2(0.f, (float)-c); break;
+      }
+      tc.push_back(v);
+    }
+  }
+  gstart.push_back((uint32_t)tzv.size());
+  h->ngroups = (uint32_t)gx.size();
+  if (upload(&h->d_gx, gx) || upload(&h->d_gstart, gstart) || upload(&h->d_tz, tzv) ||
+      upload(&h->d_tc, tc))
+    return TQSV_EDEVICE;
+  return 0;
+}
+
+// <P_t> from the raw sums
+double term_value(const tqsv_ham* h, int t) {
+  const uint64_t xm = t < h->nterms ? h->x[t] : 0ull;
+  const uint64_t zm = t < h->nterms ? h->z[t] : 0ull;
+  const int k = popc(xm & zm) & 3;
+  const int s = h->tslot[t];
+  if (h->tmode[t] == 0) {
+    const double S = h->hout[s];
+    if (!xm) return S;
+    static const double f[4] = {2.0, -2.0, -2.0, 2.0};
+    return f[k] * S;
+  }
+  // Re((-i)^k S)
+  const double re = h->hout[s], im = h->hout[s + 1];
+  switch (k) {
+    case 0: return re;
+    case 1: return im;
+    case 2: return -re;
+    default: return -im;
+  }
+}
+
+int run_expval(tqsv_ham* h, const float2* psi, hipStream_t st) {
+  const int n = h->n;
+  for (const PassPlan& pp : h->passes) {
+    PassLaunch pl;
+    pl.psi = const_cast<float2*>(psi);
+    pl.lam = nullptr;
+    pl.prog = h->dprog + pp.prog_off;
+    pl.partials = h->dpartials;
+    pl.ntiles = pp.ntiles;
+    pl.grid = pp.grid;
+    pl.lds_bytes = pp.lds;
+    TQSV_HIP(launch_pass(pp.tb, false, n > 32, pl, st));
+    TQSV_HIP(k_final_sum(h->dpartials, pp.grid, pp.nred, h->dout + pp.red_base, st));
+  }
+  if (!h->small_terms.empty())
+    TQSV_HIP(k_pauli_dot_terms(psi, psi, n, (uint32_t)h->small_terms.size(), h->d_sx, h->d_sz,
+                               h->dout + h->small_base, st));
+  for (int t : h->big_terms) {
+    TQSV_HIP(k_pauli_dot(psi, psi, n, h->x[t], h->z[t], h->dpartials, st));
+    TQSV_HIP(k_final_sum(h->dpartials, reduce_grid(n), 2, h->dout + h->tslot[t], st));
+  }
+  TQSV_HIP(hipMemcpyAsync(h->hout.data(), h->dout, h->nout * sizeof(double),
+                          hipMemcpyDeviceToHost, st));
+  TQSV_HIP(hipStreamSynchronize(st));
+  return 0;
+}
+
+}  // namespace
+
+// ---- plan build helpers shared with tqsv_abi.cpp ---------------------------
+namespace tqsv {
+
+int plan_upload(uint32_t** dptr, size_t* cap, const std::vector<uint32_t>& prog, hipStream_t st) {
+  const size_t need = std::max<size_t>(prog.size(), 1);
+  if (*cap < need) {
+    if (*dptr) TQSV_HIP(hipFree(*dptr));
+    *dptr = nullptr;
+    TQSV_HIP(hipMalloc(reinterpret_cast<void**>(dptr), need * sizeof(uint32_t)));
+    *cap = need;
+  }
+  if (!prog.empty()) {
+    TQSV_HIP(hipMemcpyAsync(*dptr, prog.data(), prog.size() * sizeof(uint32_t),
+                            hipMemcpyHostToDevice, st));
+    TQSV_HIP(hipStreamSynchronize(st));
+  }
+  return 0;
+}
+
+int plan_build_forward(tqsv_plan* p, const tqsv_op* ops, int nops, hipStream_t st) {
+  std::vector<Gate> g;
+  int rc = gates_from_ops(p->n, ops, nops, g);
+  if (rc) return rc;
+  merge_and_classify(g);
+  std::vector<PassPlan> passes;
+  std::vector<uint32_t> prog;
+  uint32_t red_total = 0;
+  rc = schedule(g, p->so, passes, prog, red_total);
+  if (rc) return rc;
+  p->max_param = -1;
+  for (const Gate& x : g) p->max_param = std::max(p->max_param, x.param);
+  p->gates.swap(g);
+  p->passes.swap(passes);
+  p->prog.swap(prog);
+  p->adj_built = false;
+  return plan_upload(&p->dprog, &p->dprog_cap, p->prog, st);
+}
+
+static int ensure(double** d, size_t* cap, size_t need) {
+  need = std::max<size_t>(need, 1);
+  if (*cap >= need) return 0;
+  if (*d) TQSV_HIP(hipFree(*d));
+  *d = nullptr;
+  TQSV_HIP(hipMalloc(reinterpret_cast<void**>(d), need * sizeof(double)));
+  *cap = need;
+  return 0;
+}
+
+int plan_build_adjoint(tqsv_plan* p, hipStream_t st) {
+  if (p->adj_built) return 0;
+  std::vector<Gate> r;
+  for (int i = (int)p->gates.size() - 1; i >= 0; --i) {
+    const Gate& g = p->gates[i];
+    if (g.param >= 0) {
+      Gate red;
+      red.kind = GK_RED;
+      red.x = g.gx;
+      red.z = g.gz;
+      red.k = popc(g.gx & g.gz) & 3;
+      red.param = g.param;
+      red.gcoef = g.gcoef;
+      classify_gate(red);
+      r.push_back(red);
+    }
+    r.push_back(dagger(g));
+  }
+  SchedOpts so = p->so;
+  so.dual = true;
+  std::vector<PassPlan> passes;
+  std::vector<uint32_t> prog;
+  uint32_t red_total = 0;
+  int rc = schedule(r, so, passes, prog, red_total);
+  if (rc) return rc;
+  p->rgates.swap(r);
+  p->rpasses.swap(passes);
+  p->rprog.swap(prog);
+  p->nred_adj = (int)red_total;
+  rc = plan_upload(&p->drprog, &p->drprog_cap, p->rprog, st);
+  if (rc) return rc;
+  size_t pcap = (size_t)reduce_grid(p->n) * 2;
+  for (const PassPlan& pp : p->rpasses)
+    if (pp.tile) pcap = std::max(pcap, (size_t)pp.grid * pp.nred);
+  rc = ensure(&p->dpartials, &p->dpartials_cap, pcap);
+  if (rc) return rc;
+  rc = ensure(&p->dred, &p->dred_cap, (size_t)red_total + 2);
+  if (rc) return rc;
+  p->adj_built = true;
+  return 0;
+}
+
+// reverse sweep + gradient assembly; scale = 1 for energies, 1/2 for VJPs
+static int reverse_and_collect(tqsv_plan* p, float2* psi, float2* lam, int nparams, double* grad,
+                               double scale, double* hred, hipStream_t st) {
+  int rc = run_passes(p->rpasses, p->rgates, p->n, p->drprog, psi, lam, p->dpartials, p->dred, st);
+  if (rc) return rc;
+  const size_t nr = (size_t)p->nred_adj + 2;
+  TQSV_HIP(hipMemcpyAsync(hred, p->dred, nr * sizeof(double), hipMemcpyDeviceToHost, st));
+  TQSV_HIP(hipStreamSynchronize(st));
+  for (int i = 0; i < nparams; ++i) grad[i] = 0.0;
+  for (const Gate& g : p->rgates) {
+    if (g.kind != GK_RED) continue;
+    const double re = hred[g.red_slot], im = hred[g.red_slot + 1];
+    double v;
+    switch (g.k) {  // Im((-i)^k S)
+      case 0: v = im; break;
+      case 1: v = -re; break;
+      case 2: v = -im; break;
+      default: v = re; break;
+    }
+    grad[g.param] += scale * g.gcoef * v;
+  }
+  return 0;
+}
+
+}  // namespace tqsv
+
+// ---- C ABI -----------------------------------------------------------------
+extern "C" {
+
+int tqsv_ham_create(int n, int nterms, const uint64_t* xmask, const uint64_t* zmask,
+                    const double* coeff, tqsv_ham** out) {
+  try {
+    if (!out) return fail(TQSV_EINVAL, "tqsv_ham_create: out is NULL");
+    *out = nullptr;
+    if (n < 1 || n > 34) return fail(TQSV_EINVAL, "tqsv_ham_create: n must be in [1, 34]");
+    if (nterms < 0 || (nterms > 0 && (!xmask || !zmask || !coeff)))
+      return fail(TQSV_EINVAL, "tqsv_ham_create: bad term arrays");
+    const uint64_t full = lowmask(n);
+    auto h = std::make_unique<tqsv_ham>();
+    h->n = n;
+    h->nterms = nterms;
+    for (int t = 0; t < nterms; ++t) {
+      if ((xmask[t] | zmask[t]) & ~full)
+        return fail(TQSV_EINVAL, "tqsv_ham_create: term " + std::to_string(t) + " acts outside n qubits");
+      h->x.push_back(xmask[t]);
+      h->z.push_back(zmask[t]);
+      h->coeff.push_back(coeff[t]);
+    }
+    int rc = plan_expval(h.get());
+    if (rc) return rc;
+    rc = plan_apply(h.get());
+    if (rc) return rc;
+    *out = h.release();
+    return 0;
+  } catch (const std::bad_alloc&) {
+    return fail(TQSV_ENOMEM, "tqsv_ham_create: out of host memory");
+  } catch (const std::exception& e) {
+    return fail(TQSV_EINVAL, std::string("tqsv_ham_create: ") + e.what());
+  }
+}
+
+int tqsv_ham_expval(tqsv_ham* h, const void* psi, double* terms, double* energy, double* norm2,
+                    void* stream) {
+  if (!h || !psi) return fail(TQSV_EINVAL, "tqsv_ham_expval: NULL argument");
+  int rc = run_expval(h, reinterpret_cast<const float2*>(psi), reinterpret_cast<hipStream_t>(stream));
+  if (rc) return rc;
+  double e = 0.0;
+  for (int t = 0; t < h->nterms; ++t) {
+    const double v = term_value(h, t);
+    if (terms) terms[t] = v;
+    e += h->coeff[t] * v;
+  }
+  if (energy) *energy = e;
+  if (norm2) *norm2 = term_value(h, h->nterms);
+  return 0;
+}
+
+int tqsv_ham_apply(tqsv_ham* h, const void* psi, void* out, void* stream) {
+  if (!h || !psi || !out) return fail(TQSV_EINVAL, "tqsv_ham_apply: NULL argument");
+  if (psi == out) return fail(TQSV_EINVAL, "tqsv_ham_apply: out must not alias psi");
+  TQSV_HIP(k_ham_apply(reinterpret_cast<const float2*>(psi), reinterpret_cast<float2*>(out), h->n,
+                       h->ngroups, h->d_gx, h->d_gstart, h->d_tz, h->d_tc,
+                       reinterpret_cast<hipStream_t>(stream)));
+  return 0;
+}
+
+int tqsv_ham_get_info(const tqsv_ham* h, tqsv_ham_info* info) {
+  if (!h || !info) return fail(TQSV_EINVAL, "tqsv_ham_get_info: NULL argument");
+  std::memset(info, 0, sizeof(*info));
+  info->n = h->n;
+  info->nterms = h->nterms;
+  int phases = 0;
+  for (const PassPlan& pp : h->passes) phases += (int)h->prog[pp.prog_off + PH_NPHASE];
+  info->passes = (int32_t)(h->passes.size() + h->big_terms.size() + (h->small_terms.empty() ? 0 : 1));
+  info->phases = phases;
+  std::map<uint64_t, int> g;
+  for (int t = 0; t < h->nterms; ++t) g[h->x[t]] = 1;
+  info->flip_groups = (int32_t)g.size();
+  info->unfused_terms = (int32_t)(h->big_terms.size() + h->small_terms.size());
+  return 0;
+}
+
+void tqsv_ham_destroy(tqsv_ham* h) { delete h; }
+
+int tqsv_expval(const void* psi, int n, int nterms, const uint64_t* xmask, const uint64_t* zmask,
+                const double* coeff, double* terms, double* energy, double* norm2, void* stream) {
+  tqsv_ham* h = nullptr;
+  int rc = tqsv_ham_create(n, nterms, xmask, zmask, coeff, &h);
+  if (rc) return rc;
+  rc = tqsv_ham_expval(h, psi, terms, energy, norm2, stream);
+  tqsv_ham_destroy(h);
+  return rc;
+}
+
+int tqsv_adjoint(tqsv_plan* p, tqsv_ham* h, void* psi, void* lam, int nparams, double* grad,
+                 double* energy, void* stream) {
+  try {
+    if (!p || !h || !psi || !lam || (nparams > 0 && !grad))
+      return fail(TQSV_EINVAL, "tqsv_adjoint: NULL argument");
+    if (p->n != h->n) return fail(TQSV_EINVAL, "tqsv_adjoint: plan and Hamiltonian sizes differ");
+    if (nparams <= p->max_param)
+      return fail(TQSV_EINVAL, "tqsv_adjoint: nparams smaller than the largest parameter index + 1");
+    if (psi == lam) return fail(TQSV_EINVAL, "tqsv_adjoint: lam must not alias psi");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    int rc = plan_build_adjoint(p, st);
+    if (rc) return rc;
+    float2* a = reinterpret_cast<float2*>(psi);
+    float2* b = reinterpret_cast<float2*>(lam);
+    rc = run_passes(p->passes, p->gates, p->n, p->dprog, a, nullptr, nullptr, nullptr, st);
+    if (rc) return rc;
+    TQSV_HIP(k_ham_apply(a, b, h->n, h->ngroups, h->d_gx, h->d_gstart, h->d_tz, h->d_tc, st));
+    // E = Re <psi|H psi>
+    TQSV_HIP(k_pauli_dot(a, b, p->n, 0, 0, p->dpartials, st));
+    TQSV_HIP(k_final_sum(p->dpartials, reduce_grid(p->n), 2, p->dred + p->nred_adj, st));
+    std::vector<double> hred((size_t)p->nred_adj + 2);
+    rc = reverse_and_collect(p, a, b, nparams, grad, 1.0, hred.data(), st);
+    if (rc) return rc;
+    if (energy) *energy = hred[(size_t)p->nred_adj];
+    return 0;
+  } catch (const std::bad_alloc&) {
+    return fail(TQSV_ENOMEM, "tqsv_adjoint: out of host memory");
+  }
+}
+
+int tqsv_vjp(tqsv_plan* p, void* psi, void* lam, int nparams, double* grad, void* stream) {
+  try {
+    if (!p || !psi || !lam || (nparams > 0 && !grad)) return fail(TQSV_EINVAL, "tqsv_vjp: NULL argument");
+    if (nparams <= p->max_param)
+      return fail(TQSV_EINVAL, "tqsv_vjp: nparams smaller than the largest parameter index + 1");
+    if (psi == lam) return fail(TQSV_EINVAL, "tqsv_vjp: lam must not alias psi");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    int rc = plan_build_adjoint(p, st);
+    if (rc) return rc;
+    std::vector<double> hred((size_t)p->nred_adj + 2);
+    return reverse_and_collect(p, reinterpret_cast<float2*>(psi), reinterpret_cast<float2*>(lam),
+                               nparams, grad, 0.5, hred.data(), st);
+  } catch (const std::bad_alloc&) {
+    return fail(TQSV_ENOMEM, "tqsv_vjp: out of host memory");
+  }
+}
+
+}  // extern "C"

@@ -1,0 +1,74 @@
+// tqsv_internal.h - declarations shared by the libtqsv translation units.
+#ifndef TQSV_INTERNAL_H
+#define TQSV_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include <string>
+
+namespace tqsv {
+
+// error state (tqsv_abi.cpp)
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+
+#define TQSV_HIP(call)                                  \
+  do {                                                  \
+    hipError_t e_ = (call);                             \
+    if (e_ != hipSuccess) return ::tqsv::hip_fail(e_, #call); \
+  } while (0)
+
+// ---- tile pass kernel (tqsv_pass.hip) ----
+struct PassLaunch {
+  float2* psi;
+  float2* lam;
+  const uint32_t* prog;   // device pointer to this pass's record
+  double* partials;       // grid * nred doubles (reduction passes)
+  uint32_t ntiles;
+  uint32_t grid;
+  size_t lds_bytes;
+};
+hipError_t launch_pass(int tb, bool dual, bool idx64, const PassLaunch& pl, hipStream_t st);
+size_t pass_lds_bytes(int tb, bool dual, uint32_t nred);
+
+// ---- single-op kernels (tqsv_kernels.hip) ----
+struct Mat2f { float v[8]; };
+struct Mat4f { float v[32]; };
+
+hipError_t k_init_basis(float2* psi, int n, uint64_t index, hipStream_t st);
+hipError_t k_apply_1q(float2* psi, int n, int bit, uint64_t cmask, uint64_t cval, const Mat2f& u,
+                      hipStream_t st);
+hipError_t k_apply_2q(float2* psi, int n, int bh, int bl, uint64_t cmask, uint64_t cval,
+                      const Mat4f& u, hipStream_t st);
+hipError_t k_apply_diag(float2* psi, int n, int nbits, const int* bits, const float* phases,
+                        hipStream_t st);
+hipError_t k_pauli_rot(float2* psi, int n, uint64_t x, uint64_t z, float c, float s, int k,
+                       hipStream_t st);
+hipError_t k_project_zero(float2* psi, int n, int bit, int keep, hipStream_t st);
+hipError_t k_scale(float2* psi, int n, float re, float im, hipStream_t st);
+hipError_t k_probs(const float2* psi, int n, double* out, hipStream_t st);
+// blockwise FP64 partial sums: mode 0 = |psi|^2, 1 = |psi|^2 (-1)^{bit}, 2 = |psi|^2 [bit==keep]
+uint32_t reduce_grid(int n);
+hipError_t k_reduce_prob(const float2* psi, int n, int mode, int bit, double* partials,
+                         hipStream_t st);
+// out[r] = sum_g partials[g * nred + r] (fixed order)
+hipError_t k_final_sum(const double* partials, uint32_t ngroups, uint32_t nred, double* out,
+                       hipStream_t st);
+// partials[g*2 + {0,1}] = block sums of (-1)^{|i&z|} conj(u[i]) v[i^x]  (grid = reduce_grid(n))
+hipError_t k_pauli_dot(const float2* u, const float2* v, int n, uint64_t x, uint64_t z,
+                       double* partials, hipStream_t st);
+// out2[2t + {0,1}] = full sums of the above for terms t (one block per term; small n)
+hipError_t k_pauli_dot_terms(const float2* u, const float2* v, int n, uint32_t nterms,
+                             const uint64_t* dx, const uint64_t* dz, double* out2,
+                             hipStream_t st);
+// out = sum_k coef_k P_k psi, terms given as device arrays
+hipError_t k_ham_apply(const float2* psi, float2* out, int n, uint32_t ngroups,
+                       const uint64_t* gx, const uint32_t* gstart, const uint64_t* tz,
+                       const float2* tc, hipStream_t st);
+
+}  // namespace tqsv
+
+#endif

@@ -1,0 +1,382 @@
+// tqsv_kernels.hip - single-operation kernels and FP64 reductions.
+//
+// These serve the one-gate entry points of the C ABI (the drop-ins for the
+// reference's apply_1q/apply_2q/expect_z kernels) and the unfused launch
+// mode of a plan.  The fused tile passes live in tqsv_pass.hip.
+#include <hip/hip_runtime.h>
+
+#include "tqsv_internal.h"
+
+namespace tqsv {
+
+namespace {
+
+constexpr int BLK = 256;
+
+__device__ __forceinline__ uint64_t ins0(uint64_t x, uint32_t b) {
+  return ((x >> b) << (b + 1)) | (x & ((1ull << b) - 1ull));
+}
+
+__device__ __forceinline__ float2 cmulf(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+__device__ __forceinline__ float2 mv2(const Mat2f& u, int r, float2 x0, float2 x1) {
+  const float* m = u.v + 4 * r;
+  return make_float2(m[0] * x0.x - m[1] * x0.y + m[2] * x1.x - m[3] * x1.y,
+                     m[0] * x0.y + m[1] * x0.x + m[2] * x1.y + m[3] * x1.x);
+}
+
+uint32_t grid_for(uint64_t work) {
+  uint64_t g = (work + BLK - 1) / BLK;
+  if (g > 65535ull * 16ull) g = 65535ull * 16ull;
+  if (g == 0) g = 1;
+  return (uint32_t)g;
+}
+
+__global__ void init_kernel(float2* psi, uint64_t dim, uint64_t index) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < dim; i += stride)
+    psi[i] = make_float2(i == index ? 1.f : 0.f, 0.f);
+}
+
+// one thread per amplitude pair, controlled by (cm, cv)
+__global__ void apply_1q_kernel(float2* psi, uint64_t npairs, uint32_t bit, uint64_t cm,
+                                uint64_t cv, Mat2f u) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < npairs; t += stride) {
+    const uint64_t i0 = ins0(t, bit);
+    if ((i0 & cm) != cv) continue;
+    const uint64_t i1 = i0 | (1ull << bit);
+    const float2 x0 = psi[i0], x1 = psi[i1];
+    psi[i0] = mv2(u, 0, x0, x1);
+    psi[i1] = mv2(u, 1, x0, x1);
+  }
+}
+
+__global__ void apply_2q_kernel(float2* psi, uint64_t nquads, uint32_t bh, uint32_t bl, uint32_t blo,
+                                uint32_t bhi, uint64_t cm, uint64_t cv, Mat4f u) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nquads; t += stride) {
+    const uint64_t i00 = ins0(ins0(t, blo), bhi);
+    if ((i00 & cm) != cv) continue;
+    const uint64_t idx[4] = {i00, i00 | (1ull << bl), i00 | (1ull << bh),
+                             i00 | (1ull << bh) | (1ull << bl)};
+    float2 x[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) x[c] = psi[idx[c]];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      float yr = 0.f, yi = 0.f;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const float mr = u.v[8 * r + 2 * c], mi = u.v[8 * r + 2 * c + 1];
+        yr += mr * x[c].x - mi * x[c].y;
+        yi += mr * x[c].y + mi * x[c].x;
+      }
+      psi[idx[r]] = make_float2(yr, yi);
+    }
+  }
+}
+
+struct DiagArgs {
+  int nbits;
+  int bits[4];
+  float ph[32];
+};
+
+__global__ void apply_diag_kernel(float2* psi, uint64_t dim, DiagArgs d) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < dim; i += stride) {
+    uint32_t k = 0;
+    for (int j = 0; j < d.nbits; ++j) k = (k << 1) | (uint32_t)((i >> d.bits[j]) & 1ull);
+    psi[i] = cmulf(psi[i], make_float2(d.ph[2 * k], d.ph[2 * k + 1]));
+  }
+}
+
+// exp(-i theta/2 P): new[i] = c psi[i] - i s omega (-1)^{|(i^x)&z|} psi[i^x]
+__global__ void pauli_rot_kernel(float2* psi, uint64_t dim, uint64_t x, uint64_t z, float c,
+                                 float s, int k) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  // kappa = -i * i^k
+  const int q = (k + 3) & 3;
+  const float kr = (q == 0) ? 1.f : (q == 2 ? -1.f : 0.f);
+  const float ki = (q == 1) ? 1.f : (q == 3 ? -1.f : 0.f);
+  if (x == 0) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < dim; i += stride) {
+      const float sg = (__popcll(i & z) & 1) ? -s : s;
+      const float2 v = psi[i];
+      // (c + sg * kappa) v with kappa = -i for k = 0
+      const float2 f = make_float2(c + sg * kr, sg * ki);
+      psi[i] = cmulf(f, v);
+    }
+    return;
+  }
+  const uint64_t lb = x & (~x + 1ull);
+  const uint32_t lbit = (uint32_t)__ffsll((unsigned long long)lb) - 1u;
+  const uint64_t npairs = dim >> 1;
+  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < npairs; t += stride) {
+    const uint64_t i0 = ins0(t, lbit), i1 = i0 ^ x;
+    const float s0 = (__popcll(i0 & z) & 1) ? -s : s;
+    const float s1 = (__popcll(i1 & z) & 1) ? -s : s;
+    const float2 a0 = psi[i0], a1 = psi[i1];
+    const float2 r1 = cmulf(make_float2(kr, ki), a1), r0 = cmulf(make_float2(kr, ki), a0);
+    psi[i0] = make_float2(c * a0.x + s1 * r1.x, c * a0.y + s1 * r1.y);
+    psi[i1] = make_float2(c * a1.x + s0 * r0.x, c * a1.y + s0 * r0.y);
+  }
+}
+
+__global__ void project_kernel(float2* psi, uint64_t dim, uint32_t bit, uint32_t keep) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < dim; i += stride)
+    if (((i >> bit) & 1ull) != keep) psi[i] = make_float2(0.f, 0.f);
+}
+
+__global__ void scale_kernel(float2* psi, uint64_t dim, float re, float im) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < dim; i += stride)
+    psi[i] = cmulf(psi[i], make_float2(re, im));
+}
+
+__global__ void probs_kernel(const float2* psi, uint64_t dim, double* out) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < dim; i += stride) {
+    const double re = psi[i].x, im = psi[i].y;
+    out[i] = re * re + im * im;
+  }
+}
+
+__device__ double block_sum(double v) {
+  __shared__ double red[BLK / 64];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) red[wv] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < BLK / 64; ++i) s += red[i];
+  return s;
+}
+
+__global__ __launch_bounds__(BLK) void reduce_prob_kernel(const float2* psi, uint64_t dim, int mode,
+                                                          uint32_t bit, double* partials) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  double acc = 0.0;
+  float facc = 0.f;
+  int cnt = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < dim; i += stride) {
+    const float2 v = psi[i];
+    float p = v.x * v.x + v.y * v.y;
+    const uint32_t b = (uint32_t)((i >> bit) & 1ull);
+    if (mode == 1) p = b ? -p : p;
+    if (mode == 2) p = b ? p : 0.f;
+    facc += p;
+    if (++cnt == 64) {  // bounded FP32 runs, accumulated in FP64
+      acc += (double)facc;
+      facc = 0.f;
+      cnt = 0;
+    }
+  }
+  acc += (double)facc;
+  const double s = block_sum(acc);
+  if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(BLK) void final_sum_kernel(const double* partials, uint32_t ngroups,
+                                                        uint32_t nred, double* out) {
+  const uint32_t r = blockIdx.x;
+  double acc = 0.0;
+  for (uint32_t g = threadIdx.x; g < ngroups; g += BLK) acc += partials[(size_t)g * nred + r];
+  const double s = block_sum(acc);
+  if (threadIdx.x == 0) out[r] = s;
+}
+
+// block partial sums of (-1)^{|i&z|} conj(u[i]) v[i^x]
+__global__ __launch_bounds__(BLK) void pauli_dot_kernel(const float2* u, const float2* v, uint64_t dim,
+                                                        uint64_t x, uint64_t z, double* partials) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  double ar = 0.0, ai = 0.0;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < dim; i += stride) {
+    const float2 a = u[i], b = v[i ^ x];
+    double pr = (double)a.x * b.x + (double)a.y * b.y;
+    double pi = (double)a.x * b.y - (double)a.y * b.x;
+    if (__popcll(i & z) & 1) {
+      pr = -pr;
+      pi = -pi;
+    }
+    ar += pr;
+    ai += pi;
+  }
+  const double sr = block_sum(ar);
+  __syncthreads();
+  const double si = block_sum(ai);
+  if (threadIdx.x == 0) {
+    partials[2 * blockIdx.x] = sr;
+    partials[2 * blockIdx.x + 1] = si;
+  }
+}
+
+// one block per term: full sums for small states
+__global__ __launch_bounds__(BLK) void pauli_dot_terms_kernel(const float2* u, const float2* v,
+                                                              uint64_t dim, const uint64_t* dx,
+                                                              const uint64_t* dz, double* out2) {
+  const uint32_t t = blockIdx.x;
+  const uint64_t x = dx[t], z = dz[t];
+  double ar = 0.0, ai = 0.0;
+  for (uint64_t i = threadIdx.x; i < dim; i += BLK) {
+    const float2 a = u[i], b = v[i ^ x];
+    double pr = (double)a.x * b.x + (double)a.y * b.y;
+    double pi = (double)a.x * b.y - (double)a.y * b.x;
+    if (__popcll(i & z) & 1) {
+      pr = -pr;
+      pi = -pi;
+    }
+    ar += pr;
+    ai += pi;
+  }
+  const double sr = block_sum(ar);
+  __syncthreads();
+  const double si = block_sum(ai);
+  if (threadIdx.x == 0) {
+    out2[2 * t] = sr;
+    out2[2 * t + 1] = si;
+  }
+}
+
+__global__ void ham_apply_kernel(const float2* psi, float2* out, uint64_t dim, uint32_t ngroups,
+                                 const uint64_t* gx, const uint32_t* gstart, const uint64_t* tz,
+                                 const float2* tc) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < dim; i += stride) {
+    float ar = 0.f, ai = 0.f;
+    for (uint32_t g = 0; g < ngroups; ++g) {
+      const uint64_t j = i ^ gx[g];
+      const float2 v = psi[j];
+      float fr = 0.f, fi = 0.f;
+      for (uint32_t t = gstart[g]; t < gstart[g + 1]; ++t) {
+        const float sg = (__popcll(j & tz[t]) & 1) ? -1.f : 1.f;
+        fr += sg * tc[t].x;
+        fi += sg * tc[t].y;
+      }
+      ar += fr * v.x - fi * v.y;
+      ai += fr * v.y + fi * v.x;
+    }
+    out[i] = make_float2(ar, ai);
+  }
+}
+
+}  // namespace
+
+hipError_t k_init_basis(float2* psi, int n, uint64_t index, hipStream_t st) {
+  const uint64_t dim = 1ull << n;
+  hipLaunchKernelGGL(init_kernel, dim3(grid_for(dim)), dim3(BLK), 0, st, psi, dim, index);
+  return hipGetLastError();
+}
+
+hipError_t k_apply_1q(float2* psi, int n, int bit, uint64_t cmask, uint64_t cval, const Mat2f& u,
+                      hipStream_t st) {
+  const uint64_t np = 1ull << (n - 1);
+  hipLaunchKernelGGL(apply_1q_kernel, dim3(grid_for(np)), dim3(BLK), 0, st, psi, np, (uint32_t)bit,
+                     cmask, cval, u);
+  return hipGetLastError();
+}
+
+hipError_t k_apply_2q(float2* psi, int n, int bh, int bl, uint64_t cmask, uint64_t cval,
+                      const Mat4f& u, hipStream_t st) {
+  const uint64_t nq = 1ull << (n - 2);
+  const uint32_t blo = (uint32_t)(bh < bl ? bh : bl), bhi = (uint32_t)(bh < bl ? bl : bh);
+  hipLaunchKernelGGL(apply_2q_kernel, dim3(grid_for(nq)), dim3(BLK), 0, st, psi, nq, (uint32_t)bh,
+                     (uint32_t)bl, blo, bhi, cmask, cval, u);
+  return hipGetLastError();
+}
+
+hipError_t k_apply_diag(float2* psi, int n, int nbits, const int* bits, const float* phases,
+                        hipStream_t st) {
+  DiagArgs d{};
+  d.nbits = nbits;
+  for (int j = 0; j < nbits; ++j) d.bits[j] = bits[j];
+  for (int k = 0; k < (2 << nbits); ++k) d.ph[k] = phases[k];
+  const uint64_t dim = 1ull << n;
+  hipLaunchKernelGGL(apply_diag_kernel, dim3(grid_for(dim)), dim3(BLK), 0, st, psi, dim, d);
+  return hipGetLastError();
+}
+
+hipError_t k_pauli_rot(float2* psi, int n, uint64_t x, uint64_t z, float c, float s, int k,
+                       hipStream_t st) {
+  const uint64_t dim = 1ull << n;
+  hipLaunchKernelGGL(pauli_rot_kernel, dim3(grid_for(x ? dim / 2 : dim)), dim3(BLK), 0, st, psi,
+                     dim, x, z, c, s, k);
+  return hipGetLastError();
+}
+
+hipError_t k_project_zero(float2* psi, int n, int bit, int keep, hipStream_t st) {
+  const uint64_t dim = 1ull << n;
+  hipLaunchKernelGGL(project_kernel, dim3(grid_for(dim)), dim3(BLK), 0, st, psi, dim,
+                     (uint32_t)bit, (uint32_t)keep);
+  return hipGetLastError();
+}
+
+hipError_t k_scale(float2* psi, int n, float re, float im, hipStream_t st) {
+  const uint64_t dim = 1ull << n;
+  hipLaunchKernelGGL(scale_kernel, dim3(grid_for(dim)), dim3(BLK), 0, st, psi, dim, re, im);
+  return hipGetLastError();
+}
+
+hipError_t k_probs(const float2* psi, int n, double* out, hipStream_t st) {
+  const uint64_t dim = 1ull << n;
+  hipLaunchKernelGGL(probs_kernel, dim3(grid_for(dim)), dim3(BLK), 0, st, psi, dim, out);
+  return hipGetLastError();
+}
+
+uint32_t reduce_grid(int n) {
+  const uint64_t dim = 1ull << n;
+  uint64_t g = (dim + BLK * 16 - 1) / (BLK * 16);
+  if (g > 2048) g = 2048;
+  if (g == 0) g = 1;
+  return (uint32_t)g;
+}
+
+hipError_t k_reduce_prob(const float2* psi, int n, int mode, int bit, double* partials,
+                         hipStream_t st) {
+  const uint64_t dim = 1ull << n;
+  hipLaunchKernelGGL(reduce_prob_kernel, dim3(reduce_grid(n)), dim3(BLK), 0, st, psi, dim, mode,
+                     (uint32_t)bit, partials);
+  return hipGetLastError();
+}
+
+hipError_t k_final_sum(const double* partials, uint32_t ngroups, uint32_t nred, double* out,
+                       hipStream_t st) {
+  if (nred == 0) return hipSuccess;
+  hipLaunchKernelGGL(final_sum_kernel, dim3(nred), dim3(BLK), 0, st, partials, ngroups, nred, out);
+  return hipGetLastError();
+}
+
+hipError_t k_pauli_dot(const float2* u, const float2* v, int n, uint64_t x, uint64_t z,
+                       double* partials, hipStream_t st) {
+  const uint64_t dim = 1ull << n;
+  hipLaunchKernelGGL(pauli_dot_kernel, dim3(reduce_grid(n)), dim3(BLK), 0, st, u, v, dim, x, z,
+                     partials);
+  return hipGetLastError();
+}
+
+hipError_t k_pauli_dot_terms(const float2* u, const float2* v, int n, uint32_t nterms,
+                             const uint64_t* dx, const uint64_t* dz, double* out2,
+                             hipStream_t st) {
+  if (nterms == 0) return hipSuccess;
+  const uint64_t dim = 1ull << n;
+  hipLaunchKernelGGL(pauli_dot_terms_kernel, dim3(nterms), dim3(BLK), 0, st, u, v, dim, dx, dz,
+                     out2);
+  return hipGetLastError();
+}
+
+hipError_t k_ham_apply(const float2* psi, float2* out, int n, uint32_t ngroups, const uint64_t* gx,
+                       const uint32_t* gstart, const uint64_t* tz, const float2* tc,
+                       hipStream_t st) {
+  const uint64_t dim = 1ull << n;
+  hipLaunchKernelGGL(ham_apply_kernel, dim3(grid_for(dim)), dim3(BLK), 0, st, psi, out, dim,
+                     ngroups, gx, gstart, tz, tc);
+  return hipGetLastError();
+}
+
+}  // namespace tqsv

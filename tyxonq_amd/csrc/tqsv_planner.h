@@ -1,0 +1,121 @@
+// tqsv_planner.h - host-side gate model and pass/phase scheduler.
+#ifndef TQSV_PLANNER_H
+#define TQSV_PLANNER_H
+
+#include <array>
+#include <complex>
+#include <cstdint>
+#include <vector>
+
+#include "../../include/tqsv.h"
+#include "tqsv_internal.h"
+#include "tqsv_prog.h"
+
+namespace tqsv {
+
+using cd = std::complex<double>;
+
+enum GateKind : int {
+  GK_MAT1 = 1,   // 2x2 on t0, controls (cmask, cval)
+  GK_MAT2 = 2,   // 4x4 on (t0 msb, t1), controls
+  GK_DIAG2 = 3,  // diagonal d[2*bit(t0)+bit(t1)] (t0 == t1 for one bit)
+  GK_PROT = 4,   // exp(-i theta/2 P)
+  GK_RED = 5,    // cross reduction <lambda| P |psi> (adjoint sweeps)
+};
+
+struct Gate {
+  int kind = 0;
+  int t0 = -1, t1 = -1;
+  uint64_t cmask = 0, cval = 0;
+  uint64_t x = 0, z = 0;
+  int k = 0;          // |x & z| mod 4
+  double c = 1.0, s = 0.0;
+  cd m[16];
+  // bit usage for commutation: nb = used non-diagonally, db = used only diagonally
+  uint64_t nb = 0, db = 0;
+  int cost = 0;
+  // adjoint bookkeeping
+  int param = -1;
+  uint64_t gx = 0, gz = 0;
+  double gcoef = 0.0;
+  int red_slot = -1;
+  int src = -1;
+};
+
+// Convert ABI ops into gates (validates).  Returns 0 or an error code.
+int gates_from_ops(int n, const tqsv_op* ops, int nops, std::vector<Gate>& out);
+// Merge runs of uncontrolled single-bit gates (param < 0) and classify.
+void merge_and_classify(std::vector<Gate>& g);
+// Inverse gate (same usage).
+Gate dagger(const Gate& g);
+
+struct PassPlan {
+  bool tile = false;     // tile pass; else a single-op launch of `single`
+  int tb = 0;
+  bool dual = false;
+  uint32_t L = 0, m = 0;
+  uint32_t H[MAXH] = {};
+  std::vector<std::vector<int>> phases;   // gate indices per phase
+  std::vector<std::array<uint32_t, RB>> Rpos;  // register positions per phase, ascending
+  int single = -1;
+  uint32_t nred = 0;      // reduction slots of this pass
+  uint32_t red_base = 0;  // first slot in the plan's reduction output
+  // emitted program
+  size_t prog_off = 0;   // words
+  size_t prog_words = 0;
+  uint32_t ntiles = 0, grid = 0;
+  size_t lds = 0;
+};
+
+struct SchedOpts {
+  int n = 0;
+  int tile_bits = 13;
+  int min_low = 5;
+  int max_cost = 96;
+  bool fuse = true;
+  bool dual = false;
+};
+
+// Classify one gate (usage bits, diagonal forms, cost).
+void classify_gate(Gate& g);
+
+// Plan passes over `gates` (in order).  Appends the programs to `prog`.
+// Reduction gates get their output slot (gates[i].red_slot, two doubles:
+// re, im) allocated from `red_total`.
+int schedule(std::vector<Gate>& gates, const SchedOpts& so, std::vector<PassPlan>& passes,
+             std::vector<uint32_t>& prog, uint32_t& red_total);
+
+// Run planned passes (device program at dprog; partial-sum scratch for reductions).
+int run_passes(const std::vector<PassPlan>& passes, const std::vector<Gate>& gates, int n,
+               const uint32_t* dprog, float2* psi, float2* lam, double* dpartials,
+               double* dred_out, hipStream_t st);
+
+int effective_tile_bits(int n, int requested, bool dual);
+
+}  // namespace tqsv
+
+struct tqsv_plan {
+  int n = 0;
+  int nops = 0;
+  tqsv::SchedOpts so;
+  std::vector<tqsv::Gate> gates;          // forward gates after merge
+  std::vector<tqsv::PassPlan> passes;
+  std::vector<uint32_t> prog;
+  uint32_t* dprog = nullptr;
+  size_t dprog_cap = 0;
+  // adjoint (built lazily)
+  bool adj_built = false;
+  std::vector<tqsv::Gate> rgates;         // reverse sweep items
+  std::vector<tqsv::PassPlan> rpasses;
+  std::vector<uint32_t> rprog;
+  uint32_t* drprog = nullptr;
+  size_t drprog_cap = 0;
+  int nred_adj = 0;
+  double* dpartials = nullptr;
+  size_t dpartials_cap = 0;
+  double* dred = nullptr;
+  size_t dred_cap = 0;
+  int max_param = -1;
+};
+
+#endif
