@@ -177,6 +177,13 @@ int tqsv_plan_get_info(const tqsv_plan* plan, tqsv_plan_info* info);
 int tqsv_plan_inspect(int n, const tqsv_op* ops, int nops, const tqsv_plan_opts* opts,
                       tqsv_plan_info* info);
 
+/* Plan on the host and execute the pass programs on a HOST complex64 state
+ * (2^n interleaved float pairs, in place, n <= 26) with the device kernel's
+ * index arithmetic and step semantics.  For testing the planner and the
+ * program format without a GPU. */
+int tqsv_plan_emulate(int n, const tqsv_op* ops, int nops, const tqsv_plan_opts* opts,
+                      float* state);
+
 /* Run the planned circuit on psi in place. */
 int tqsv_plan_run(tqsv_plan* plan, void* psi, void* stream);
 

@@ -58,20 +58,35 @@ def sources() -> tuple[list[Path], list[Path]]:
     return hip, cpp
 
 
-def build(verbose: bool = False) -> Path:
+def _stale(src: Path, obj: Path, headers: list[Path]) -> bool:
+    if not obj.exists():
+        return True
+    t = obj.stat().st_mtime
+    return src.stat().st_mtime > t or any(h.stat().st_mtime > t for h in headers)
+
+
+def build(verbose: bool = False, incremental: bool = False) -> Path:
+    """Compile every source (or, with incremental=True, only those older than
+    their object or any header) and link libtqsv.so."""
     hipcc = str(ROCM / "bin" / "hipcc")
     OBJ.mkdir(exist_ok=True)
     hip, cpp = sources()
+    headers = sorted(CSRC.glob("*.h")) + sorted((PKG.parent / "include").glob("*.h"))
     jobs = []
     for src in hip:
-        jobs.append([hipcc, *HIP_FLAGS, "-c", str(src), "-o", str(OBJ / (src.stem + ".o"))])
+        obj = OBJ / (src.stem + ".o")
+        if not incremental or _stale(src, obj, headers):
+            jobs.append([hipcc, *HIP_FLAGS, "-c", str(src), "-o", str(obj)])
     for src in cpp:
-        jobs.append(["g++", *CXX_FLAGS, "-c", str(src), "-o", str(OBJ / (src.stem + ".o"))])
+        obj = OBJ / (src.stem + ".o")
+        if not incremental or _stale(src, obj, headers):
+            jobs.append(["g++", *CXX_FLAGS, "-c", str(src), "-o", str(obj)])
     workers = max(1, min(len(jobs), int(os.environ.get("MAX_JOBS", os.cpu_count() or 4))))
-    with ThreadPoolExecutor(max_workers=workers) as ex:
-        for cmd, _ in zip(jobs, ex.map(_run, jobs)):
-            if verbose:
-                print(" ".join(cmd[:1] + cmd[-3:]))
+    if jobs:
+        with ThreadPoolExecutor(max_workers=workers) as ex:
+            for cmd, _ in zip(jobs, ex.map(_run, jobs)):
+                if verbose:
+                    print(" ".join(cmd[:1] + cmd[-3:]))
     objs = [str(OBJ / (s.stem + ".o")) for s in hip + cpp]
     tmp = OUT.with_name(OUT.name + ".tmp")
     _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *objs])
@@ -80,5 +95,5 @@ def build(verbose: bool = False) -> Path:
 
 
 if __name__ == "__main__":
-    p = build(verbose="-v" in sys.argv)
+    p = build(verbose="-v" in sys.argv, incremental="--incremental" in sys.argv)
     print(p)

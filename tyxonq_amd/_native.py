@@ -105,6 +105,7 @@ _SIGS = {
     "tqsv_plan_update": (_I, [_P, _P, _I, _P]),
     "tqsv_plan_get_info": (_I, [_P, ctypes.POINTER(PlanInfo)]),
     "tqsv_plan_inspect": (_I, [_I, _P, _I, ctypes.POINTER(PlanOpts), ctypes.POINTER(PlanInfo)]),
+    "tqsv_plan_emulate": (_I, [_I, _P, _I, ctypes.POINTER(PlanOpts), _P]),
     "tqsv_plan_run": (_I, [_P, _P, _P]),
     "tqsv_plan_destroy": (None, [_P]),
     "tqsv_apply_fused": (_I, [_P, _I, _P, _I, ctypes.POINTER(PlanOpts), _P]),
@@ -317,6 +318,17 @@ def inspect(n: int, ops: np.ndarray, *, tile_bits: int = 0, min_low_bits: int = 
     _check(load().tqsv_plan_inspect(int(n), ops.ctypes.data if len(ops) else None, len(ops),
                                     ctypes.byref(o), ctypes.byref(inf)))
     return {f: getattr(inf, f) for f, _ in PlanInfo._fields_}
+
+
+def emulate(n: int, ops: np.ndarray, state: np.ndarray, *, tile_bits: int = 0,
+            min_low_bits: int = 0, max_cost: int = 0, fuse: bool = True) -> np.ndarray:
+    """Run the planned passes on a host complex64 copy of `state` (no GPU)."""
+    ops = np.ascontiguousarray(ops, dtype=OP_DTYPE)
+    st = np.ascontiguousarray(np.asarray(state, dtype=np.complex64).copy())
+    o = _opts(tile_bits, min_low_bits, max_cost, fuse)
+    _check(load().tqsv_plan_emulate(int(n), ops.ctypes.data if len(ops) else None, len(ops),
+                                    ctypes.byref(o), st.ctypes.data))
+    return st
 
 
 class Plan:

@@ -103,12 +103,25 @@ int plan_expval(tqsv_ham* h) {
   if (tiles) {
     const int TB = T - RB;
     const int Lmin = std::min(5, T - RB);
-    // most constrained groups first
-    std::vector<uint64_t> order;
-    for (auto& kv : groups) order.push_back(kv.first);
-    std::stable_sort(order.begin(), order.end(), [&](uint64_t a, uint64_t b) {
-      return popc(a & ~lowmask(Lmin)) > popc(b & ~lowmask(Lmin));
+    // groups split into chunks of at most NRED_MAX/2 terms, most constrained first
+    struct Chunk {
+      uint64_t x;
+      std::vector<int> terms;
+    };
+    std::vector<Chunk> chunks;
+    for (auto& kv : groups)
+      for (size_t i = 0; i < kv.second.size(); i += NRED_MAX / 2) {
+        Chunk ch;
+        ch.x = kv.first;
+        ch.terms.assign(kv.second.begin() + i,
+                        kv.second.begin() + std::min(kv.second.size(), i + NRED_MAX / 2));
+        chunks.push_back(ch);
+      }
+    std::stable_sort(chunks.begin(), chunks.end(), [&](const Chunk& a, const Chunk& b) {
+      return popc(a.x & ~lowmask(Lmin)) > popc(b.x & ~lowmask(Lmin));
     });
+    std::vector<uint64_t> order;
+    for (const Chunk& ch : chunks) order.push_back(ch.x);
     std::vector<char> done(order.size(), 0);
     size_t left = order.size();
     while (left) {
@@ -118,7 +131,7 @@ int plan_expval(tqsv_ham* h) {
       for (size_t i = 0; i < order.size(); ++i) {
         if (done[i]) continue;
         const uint64_t need = order[i] & ~S;
-        const uint32_t c = (uint32_t)groups[order[i]].size();
+        const uint32_t c = (uint32_t)chunks[i].terms.size();
         if (popc(S | need) > T || nred + c > NRED_MAX) continue;
         S |= need;
         nred += c;
@@ -131,6 +144,8 @@ int plan_expval(tqsv_ham* h) {
       }
       std::vector<int> H;
       for (uint64_t v = S & ~lowmask(L); v; v &= v - 1) H.push_back(ctz(v));
+      if ((int)H.size() > MAXH || L + (int)H.size() != T)
+        return fail(TQSV_EINVAL, "expectation planner built an invalid tile (internal error)");
       const uint64_t tileb = lowmask(L) | (S & ~lowmask(L));
       auto pos = [&](int b) {
         if (b < L) return b;
@@ -148,7 +163,7 @@ int plan_expval(tqsv_ham* h) {
       nred = 0;
       for (size_t i = 0; i < order.size(); ++i) {
         if (done[i] || (order[i] & ~tileb)) continue;
-        const uint32_t c = (uint32_t)groups[order[i]].size();
+        const uint32_t c = (uint32_t)chunks[i].terms.size();
         if (nred + c > NRED_MAX) continue;
         mine.push_back(i);
         nred += c;
@@ -207,7 +222,7 @@ int plan_expval(tqsv_ham* h) {
         for (int r = 0; r < RB; ++r) w[hdr + PS_R + r] = Rp[r];
         for (size_t i : sel) {
           const uint64_t xm = order[i];
-          const std::vector<int>& terms = groups[xm];
+          const std::vector<int>& terms = chunks[i].terms;
           const size_t at = w.size();
           w.push_back(OP_RED);
           w.push_back(slotmask(xm));

@@ -65,6 +65,8 @@ void to_diag2(Gate& g, int a, int b, const cd d[4]) {
   for (int j = 0; j < 4; ++j) g.m[j] = d[j];
   for (int j = 4; j < 16; ++j) g.m[j] = cd(0.0, 0.0);
   g.cmask = g.cval = 0;
+  g.nb = 0;
+  g.db = bitm(a) | bitm(b);
 }
 
 void classify(Gate& g) {
@@ -537,6 +539,7 @@ int schedule(std::vector<Gate>& gates, const SchedOpts& so, std::vector<PassPlan
   const int TBv = T - RB;
   int Lmin = so.min_low > 0 ? so.min_low : 5;
   if (Lmin > T - RB) Lmin = T - RB;
+  if (Lmin < T - MAXH) Lmin = T - MAXH;  // at most MAXH high bits fit the pass header
   if (Lmin < 1) Lmin = 1;
   const uint64_t full = lowmask(n);
   const int budget0 = so.max_cost > 0 ? so.max_cost : 96;
@@ -565,6 +568,7 @@ int schedule(std::vector<Gate>& gates, const SchedOpts& so, std::vector<PassPlan
     uint64_t S = lowmask(Lmin);
     uint64_t blkN = 0, blkD = 0;
     int budget = budget0;
+    bool any = false;  // the first admissible gate is taken whatever its cost
     for (int gi : rem) {
       const Gate& g = gates[gi];
       if (blkN == full) break;
@@ -573,12 +577,13 @@ int schedule(std::vector<Gate>& gates, const SchedOpts& so, std::vector<PassPlan
         continue;
       }
       const uint64_t need = g.nb & ~S;
-      if (popc(S | need) > T || g.cost > budget) {
+      if (popc(S | need) > T || (any && g.cost > budget)) {
         block(g, blkN, blkD);
         continue;
       }
       S |= need;
       budget -= g.cost;
+      any = true;
     }
     TileGeo geo;
     geo.T = T;
@@ -597,6 +602,8 @@ int schedule(std::vector<Gate>& gates, const SchedOpts& so, std::vector<PassPlan
         S |= bitm(b);
       }
     std::sort(geo.H.begin(), geo.H.end());
+    if ((int)geo.H.size() > MAXH || L < 1 || L + (int)geo.H.size() != T)
+      return fail(TQSV_EINVAL, "planner built an invalid tile (internal error)");
     const uint64_t tilebits = lowmask(L) | (S & ~lowmask(L));
     // 2) take every admissible gate with the tile fixed
     std::vector<int> chosen;
@@ -604,13 +611,15 @@ int schedule(std::vector<Gate>& gates, const SchedOpts& so, std::vector<PassPlan
     budget = budget0;
     for (int gi : rem) {
       const Gate& g = gates[gi];
-      if (conflicts(g, blkN, blkD) || !fusable(g) || (g.nb & ~tilebits) || g.cost > budget) {
+      if (conflicts(g, blkN, blkD) || !fusable(g) || (g.nb & ~tilebits) ||
+          (!chosen.empty() && g.cost > budget)) {
         block(g, blkN, blkD);
         continue;
       }
       chosen.push_back(gi);
       budget -= g.cost;
     }
+    if (chosen.empty()) return fail(TQSV_EINVAL, "planner made no progress (internal error)");
     // 3) phases
     std::vector<int> prem = chosen;
     while (!prem.empty()) {
