@@ -144,6 +144,10 @@ typedef struct tqsv_plan_opts {
   int32_t min_low_bits;  /* contiguous low bits every tile keeps; 0 = automatic */
   int32_t max_cost;      /* VALU budget per pass, ops per amplitude; 0 = automatic */
   int32_t fuse;          /* 1 = fused tile passes; 0 = one kernel launch per op */
+  int32_t jit;           /* 1 = passes run as kernels generated and compiled for this
+                            plan (hiprtc, cached per structure); -1 = the precompiled
+                            interpreting kernel; 0 = automatic (generated for n >= 20).
+                            The environment variable TQSV_JIT=0/1 overrides. */
 } tqsv_plan_opts;
 
 typedef struct tqsv_plan_info {

@@ -1,10 +1,11 @@
 """Fused tile passes (libtqsv plans) vs the CPU oracle (GPU).
 
-Covers every step type of the pass kernel: general/real/RX-like 2x2 on each
-register slot, controlled 2x2, 4x4 on slot pairs, diagonal tables and
-per-thread factors, CZ batches, mixed 2-bit diagonals, Pauli rotations with
-1..5 flip bits (and >5 through the single-op path), at tile sizes 2^11..2^14,
-with small and large per-pass budgets and 64-bit-index-free sizes.
+Covers every step kind of the precompiled pass kernel: rotations with
+deferred scale in all four forms, one-sided phases, sign flips, slot tables,
+per-thread factors and rescaling of the pending diagonal, general and
+controlled 2x2, 4x4 on slot pairs, 2-bit diagonals that are not phases, Pauli
+rotations with 1..5 flip bits (and >5 through the single-op path), at tile
+sizes 2^11..2^14, with small and large per-pass budgets.
 """
 from __future__ import annotations
 

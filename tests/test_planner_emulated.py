@@ -12,7 +12,8 @@ import pytest
 
 import tyxonq_amd as tq
 from oracle import statevector as O
-from tests.gpu_util import rand_state, random_circuit_ops
+from tests.gpu_util import (dense_apply_items, ops_from_items, rand_state, random_circuit_ops,
+                            random_matrix_items)
 from tyxonq_amd import _native as nat
 from tyxonq_amd.engine import lower
 
@@ -86,3 +87,36 @@ def test_brickwork_emulated_and_plan_limits():
     for opts in ({}, {"tile_bits": 14}, {"min_low_bits": 1}, {"max_cost": 100000}):
         info = nat.inspect(30, L30.segments[0][1], **opts)
         assert info["passes"] >= 1
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_decomposed_rotations_and_pending_diagonals(seed):
+    # arbitrary 1-qubit unitaries (incl. scaled ones), diagonal gates on any two
+    # bits (sign-only and general phases), controlled and non-unitary matrices
+    n = 12
+    rng = np.random.default_rng(77 + seed)
+    items = random_matrix_items(rng, n, 160)
+    v = rand_state(rng, n)
+    want = dense_apply_items(n, items, v)
+    ops = ops_from_items(n, items)
+    for opts in ({}, {"max_cost": 100000}, {"tile_bits": 11, "max_cost": 12}):
+        got = nat.emulate(n, ops, v.astype(np.complex64), **opts)
+        scale = np.abs(want).max()
+        np.testing.assert_allclose(got / scale, want / scale, atol=5e-5)
+
+
+def test_long_rotation_chain_rescales():
+    # hundreds of deferred-scale rotations inside one pass must not overflow
+    n = 12
+    rng = np.random.default_rng(3)
+    items = []
+    for i in range(600):
+        q = int(rng.integers(3))
+        items.append(("m1", q, O.gate_rx(rng.uniform(1.2, 1.9)) if i % 2 else O.gate_ry(1.5), 0, 0))
+        items.append(("m2", q, (q + 1) % 3, np.diag([1, 1, 1, -1])))
+    v = rand_state(rng, n)
+    want = dense_apply_items(n, items, v)
+    got = nat.emulate(n, ops_from_items(n, items), v.astype(np.complex64), max_cost=100000)
+    assert np.isfinite(got).all()
+    np.testing.assert_allclose(got, want, atol=2e-4)
+    assert abs(np.linalg.norm(got) - 1) < 1e-4

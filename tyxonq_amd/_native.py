@@ -48,6 +48,7 @@ class PlanOpts(ctypes.Structure):
         ("min_low_bits", ctypes.c_int32),
         ("max_cost", ctypes.c_int32),
         ("fuse", ctypes.c_int32),
+        ("jit", ctypes.c_int32),
     ]
 
 
@@ -305,8 +306,9 @@ def probs(psi):
 
 # ---- plans -----------------------------------------------------------------
 
-def _opts(tile_bits: int = 0, min_low_bits: int = 0, max_cost: int = 0, fuse: bool = True):
-    return PlanOpts(int(tile_bits), int(min_low_bits), int(max_cost), 1 if fuse else 0)
+def _opts(tile_bits: int = 0, min_low_bits: int = 0, max_cost: int = 0, fuse: bool = True,
+          jit: int = 0):
+    return PlanOpts(int(tile_bits), int(min_low_bits), int(max_cost), 1 if fuse else 0, int(jit))
 
 
 def inspect(n: int, ops: np.ndarray, *, tile_bits: int = 0, min_low_bits: int = 0,
@@ -335,12 +337,12 @@ class Plan:
     """A planned (fused) circuit: owns its device program."""
 
     def __init__(self, n: int, ops: np.ndarray, *, tile_bits: int = 0, min_low_bits: int = 0,
-                 max_cost: int = 0, fuse: bool = True):
+                 max_cost: int = 0, fuse: bool = True, jit: int = 0):
         load()
         self.n = int(n)
         self._ops = np.ascontiguousarray(ops, dtype=OP_DTYPE)
         self._h = ctypes.c_void_p()
-        o = _opts(tile_bits, min_low_bits, max_cost, fuse)
+        o = _opts(tile_bits, min_low_bits, max_cost, fuse, jit)
         _check(
             load().tqsv_plan_create(
                 self.n, self._ops.ctypes.data if len(self._ops) else None, len(self._ops),

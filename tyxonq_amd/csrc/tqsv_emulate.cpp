@@ -117,34 +117,65 @@ void run_steps(const uint32_t* w, uint32_t nsteps, cf* a, cf* b, bool dual, uint
         }
         break;
       }
-      case OP_DIAG: {
-        const uint32_t ntab = w[1], nthr = w[2];
-        cf f(1.f, 0.f);
-        const uint32_t* e = w + DIAG_HDR + (ntab ? DIAG_TAB : 0);
-        for (uint32_t k = 0; k < nthr; ++k, e += DIAG_ENT) {
-          const uint32_t sel = 2u * (uint32_t)((gb >> e[0]) & 1u) + (uint32_t)((gb >> e[1]) & 1u);
-          f *= cf(fw(e + 2 + 2 * sel), fw(e + 3 + 2 * sel));
-        }
-        for (int z = 0; z < narr; ++z)
+      case OP_ROT: {
+        const int K = (int)w[1];
+        const uint32_t md = w[2];
+        float t = fw(w + 3);
+        const uint64_t zs = w64(w + 4);
+        if (par64(gb & zs)) t = -t;
+        const cf I(0.f, 1.f);
+        for (int z = 0; z < narr; ++z) {
+          cf* x = arrs[z];
           for (int s = 0; s < NSLOT; ++s) {
-            cf m = f;
-            if (ntab) m = cf(fw(w + DIAG_HDR + 2 * s), fw(w + DIAG_HDR + 2 * s + 1)) * f;
-            arrs[z][s] *= m;
+            if (s & (1 << K)) continue;
+            const int s1 = s | (1 << K);
+            const cf x0 = x[s], x1 = x[s1];
+            switch (md) {
+              case 0: x[s] = x0 - t * x1; x[s1] = x1 + t * x0; break;
+              case 1: x[s] = t * x0 - x1; x[s1] = t * x1 + x0; break;
+              case 2: x[s] = x0 - I * t * x1; x[s1] = x1 - I * t * x0; break;
+              default: x[s] = t * x0 - I * x1; x[s1] = t * x1 - I * x0; break;
+            }
           }
+        }
         break;
       }
-      case OP_CZ: {
-        const uint32_t cnt = w[1];
-        for (int s = 0; s < NSLOT; ++s) {
-          const uint64_t g = gb ^ gs(s);
-          uint32_t flip = 0;
-          for (uint32_t k = 0; k < cnt; ++k) {
-            const uint64_t mask = w64(w + 2 + 3 * k);
-            flip ^= par64(g & (g >> w[4 + 3 * k]) & mask);
-          }
-          if (flip)
-            for (int z = 0; z < narr; ++z) arrs[z][s] = -arrs[z][s];
-        }
+      case OP_PH: {
+        const int K = (int)w[1];
+        cf u(fw(w + 4), fw(w + 5));
+        const uint32_t* e = w + OPH_HDR;
+        for (uint32_t i = 0; i < w[2]; ++i, e += OPH_ENT)
+          if ((gb >> e[0]) & 1u) u *= cf(fw(e + 1), fw(e + 2));
+        for (int z = 0; z < narr; ++z)
+          for (int s = 0; s < NSLOT; ++s)
+            if (s & (1 << K)) arrs[z][s] *= u;
+        break;
+      }
+      case OP_SGN: {
+        for (int z = 0; z < narr; ++z)
+          for (int s = 0; s < NSLOT; ++s)
+            if ((w[1] >> s) & 1u) arrs[z][s] = -arrs[z][s];
+        break;
+      }
+      case OP_TAB: {
+        for (int z = 0; z < narr; ++z)
+          for (int s = 0; s < NSLOT; ++s)
+            arrs[z][s] *= cf(fw(w + TAB_OFF + 2 * s), fw(w + TAB_OFF + 2 * s + 1));
+        break;
+      }
+      case OP_SCL: {
+        const float f = fw(w + 1);
+        for (int z = 0; z < narr; ++z)
+          for (int s = 0; s < NSLOT; ++s) arrs[z][s] *= f;
+        break;
+      }
+      case OP_THR: {
+        cf F(fw(w + 2), fw(w + 3));
+        const uint32_t* e = w + THR_HDR;
+        for (uint32_t i = 0; i < w[1]; ++i, e += THR_ENT)
+          if (((gb >> e[0]) & (gb >> e[1])) & 1u) F *= cf(fw(e + 2), fw(e + 3));
+        for (int z = 0; z < narr; ++z)
+          for (int s = 0; s < NSLOT; ++s) arrs[z][s] *= F;
         break;
       }
       case OP_D2: {

@@ -261,7 +261,7 @@ int plan_expval(tqsv_ham* h) {
       pp.nred = local;
       pp.ntiles = (uint32_t)(1ull << (n - T));
       pp.grid = std::min<uint32_t>(pp.ntiles, 2048u);
-      pp.lds = pass_lds_bytes(TB, false, local);
+      pp.lds = pass_lds_bytes(TB, MODE_RED, local);
       h->partials_cap = std::max(h->partials_cap, (size_t)pp.grid * local);
       nout += local;
       h->passes.push_back(pp);
@@ -360,7 +360,10 @@ int run_expval(tqsv_ham* h, const float2* psi, hipStream_t st) {
     pl.ntiles = pp.ntiles;
     pl.grid = pp.grid;
     pl.lds_bytes = pp.lds;
-    TQSV_HIP(launch_pass(pp.tb, false, n > 32, pl, st));
+    if (pp.jit_fn)
+      TQSV_HIP(jit_launch(pp, pl, st));
+    else
+      TQSV_HIP(launch_pass(pp.tb, MODE_RED, n > 32, pl, st));
     TQSV_HIP(k_final_sum(h->dpartials, pp.grid, pp.nred, h->dout + pp.red_base, st));
   }
   if (!h->small_terms.empty())
@@ -402,6 +405,8 @@ int plan_build_forward(tqsv_plan* p, const tqsv_op* ops, int nops, hipStream_t s
   int rc = gates_from_ops(p->n, ops, nops, g);
   if (rc) return rc;
   merge_and_classify(g);
+  if (g.size() == p->gates.size())
+    for (size_t i = 0; i < g.size(); ++i) g[i].rot_form = p->gates[i].rot_form;
   std::vector<PassPlan> passes;
   std::vector<uint32_t> prog;
   uint32_t red_total = 0;
@@ -409,6 +414,10 @@ int plan_build_forward(tqsv_plan* p, const tqsv_op* ops, int nops, hipStream_t s
   if (rc) return rc;
   p->max_param = -1;
   for (const Gate& x : g) p->max_param = std::max(p->max_param, x.param);
+  if (p->jit) {
+    rc = jit_prepare(passes, prog, p->n);
+    if (rc) return rc;
+  }
   p->gates.swap(g);
   p->passes.swap(passes);
   p->prog.swap(prog);
@@ -444,6 +453,8 @@ int plan_build_adjoint(tqsv_plan* p, hipStream_t st) {
     }
     r.push_back(dagger(g));
   }
+  if (r.size() == p->rgates.size())
+    for (size_t i = 0; i < r.size(); ++i) r[i].rot_form = p->rgates[i].rot_form;
   SchedOpts so = p->so;
   so.dual = true;
   std::vector<PassPlan> passes;
@@ -451,6 +462,10 @@ int plan_build_adjoint(tqsv_plan* p, hipStream_t st) {
   uint32_t red_total = 0;
   int rc = schedule(r, so, passes, prog, red_total);
   if (rc) return rc;
+  if (p->jit) {
+    rc = jit_prepare(passes, prog, p->n);
+    if (rc) return rc;
+  }
   p->rgates.swap(r);
   p->rpasses.swap(passes);
   p->rprog.swap(prog);
@@ -487,7 +502,7 @@ static int reverse_and_collect(tqsv_plan* p, float2* psi, float2* lam, int npara
       case 2: v = -im; break;
       default: v = re; break;
     }
-    grad[g.param] += scale * g.gcoef * v;
+    grad[g.param] += scale * g.gcoef * g.red_scale * v;
   }
   return 0;
 }
@@ -518,6 +533,10 @@ int tqsv_ham_create(int n, int nterms, const uint64_t* xmask, const uint64_t* zm
     }
     int rc = plan_expval(h.get());
     if (rc) return rc;
+    if (jit_policy(0, n)) {
+      rc = jit_prepare(h->passes, h->prog, n);
+      if (rc) return rc;
+    }
     rc = plan_apply(h.get());
     if (rc) return rc;
     *out = h.release();

@@ -31,8 +31,10 @@ struct PassLaunch {
   uint32_t grid;
   size_t lds_bytes;
 };
-hipError_t launch_pass(int tb, bool dual, bool idx64, const PassLaunch& pl, hipStream_t st);
-size_t pass_lds_bytes(int tb, bool dual, uint32_t nred);
+// mode: MODE_GATE / MODE_RED / MODE_DUAL (tqsv_prog.h)
+hipError_t launch_pass(int tb, int mode, bool idx64, const PassLaunch& pl, hipStream_t st);
+size_t pass_lds_bytes(int tb, int mode, uint32_t nred);
+bool pass_variant_exists(int tb, int mode, bool idx64);
 
 // ---- single-op kernels (tqsv_kernels.hip) ----
 struct Mat2f { float v[8]; };

@@ -1,0 +1,253 @@
+// tqsv_jit.cpp - tile-pass kernels generated per pass program and compiled at
+// run time with hiprtc.
+//
+// The precompiled kernel (tqsv_pass_impl.h) reads each step's kind and slot
+// bits from the program and branches to one of many code arms; with 64 live
+// slot registers every such branch costs register copies at the join, scalar
+// branches and waits on scalar loads.  For plans that are run many times the
+// step sequence of each pass is instead written out as straight-line code: a
+// generated kernel fixes the tile geometry, each phase's register positions
+// and each step's kind and slot bits, and calls the same step functions
+// (tqsv_device.h).  The numbers (angles, phases, matrices) are still read from
+// the program buffer, so the compiled code stays valid when the parameters of
+// a plan change without changing its structure; the generated source is the
+// cache key.
+#include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cstdlib>
+#include <mutex>
+#include <sstream>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "tqsv_planner.h"
+
+namespace tqsv {
+
+// header texts, embedded at build time (tqsv_jit_headers.cpp, written by _build.py)
+extern const char* const kJitHdrDevice;
+extern const char* const kJitHdrProg;
+
+namespace {
+
+struct JitKernel {
+  hipModule_t mod = nullptr;
+  hipFunction_t fn = nullptr;
+};
+
+std::mutex g_mu;
+std::unordered_map<std::string, JitKernel> g_cache;  // device id + source -> kernel
+
+const char* kKernelName = "tqsv_jit_pass";
+
+int pass_mode(uint32_t flags) {
+  if (flags & PF_DUAL) return MODE_DUAL;
+  return (flags & PF_WRITE) ? MODE_GATE : MODE_RED;
+}
+
+// Source of the kernel for one pass record (pw = the pass's program words).
+std::string gen_source(int tb, const uint32_t* pw, int n) {
+  const uint32_t flags = pw[PH_FLAGS], L = pw[PH_L], m = pw[PH_M];
+  const uint32_t nphase = pw[PH_NPHASE], nred = pw[PH_NRED];
+  const int mode = pass_mode(flags);
+  const bool write = (flags & PF_WRITE) != 0;
+  const int NT = 1 << tb, TSIZE = 1 << (tb + RB), NWAVE = std::max(NT / 64, 1);
+  std::ostringstream o;
+  o << "#include \"tqsv_device.h\"\nusing namespace tqsv;\n"
+    << "extern \"C\" __global__ void __launch_bounds__(" << NT << ") " << kKernelName
+    << "(float2* __restrict__ psi, float2* __restrict__ lam, const uint32_t* __restrict__ prog_g,"
+       " double* __restrict__ partials, uint32_t ntiles) {\n"
+    << "  typedef " << (n > 32 ? "uint64_t" : "uint32_t") << " IDX;\n"
+    << "  typedef TileMap<IDX, " << L;
+  for (uint32_t k = 0; k < m; ++k) o << ", " << pw[PH_H + k];
+  o << "> TM;\n"
+    << "  constexpr int MODE = " << mode << ";\n"
+    << "  __shared__ __align__(16) float2 tA[" << TSIZE << "];\n";
+  if (mode == MODE_DUAL)
+    o << "  __shared__ __align__(16) float2 tB[" << TSIZE << "];\n";
+  else
+    o << "  float2* tB = nullptr;\n";
+  if (nred) o << "  __shared__ double acc[" << NWAVE * nred << "];\n";
+  o << "  const cw_t prog = (cw_t)prog_g;\n"
+    << "  const uint32_t tid = threadIdx.x;\n";
+  if (nred) o << "  jit_zero_acc<" << NWAVE << ", " << nred << ">(acc, tid, " << NT << ");\n";
+  o << "  for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {\n"
+    << "    const IDX base = TM::base(tile);\n"
+    << "    __syncthreads();\n"
+    << "    jit_load_tile<" << tb << ", MODE, IDX, TM>(psi, lam, tA, tB, base, tid);\n"
+    << "    __syncthreads();\n";
+  size_t at = PH_SIZE;
+  for (uint32_t ip = 0; ip < nphase; ++ip) {
+    const uint32_t* ph = pw + at;
+    const uint32_t nsteps = ph[PS_NSTEPS];
+    o << "    {\n      PhaseGeo<IDX> G;\n      jit_geo<TM, IDX";
+    for (int j = 0; j < RB; ++j) o << ", " << ph[PS_R + j];
+    o << ">(G, base, tid);\n"
+      << "      float2 a[NSLOT], b[NSLOT];\n"
+      << "      jit_load_slots<MODE>(a, b, tA, tB, G);\n";
+    size_t ws = at + PS_SIZE;
+    for (uint32_t is = 0; is < nsteps; ++is) {
+      const uint32_t* w = pw + ws;
+      const uint32_t op = w[0] & 0xffffu, nw = w[0] >> 16;
+      const std::string P = "prog + " + std::to_string(ws);
+      switch (op) {
+        case OP_ROT:
+          o << "      jrot<" << w[1] << ", " << w[2] << ", " << ((w[4] | w[5]) ? "true" : "false")
+            << ", MODE>(a, b, " << P << ", G);\n";
+          break;
+        case OP_SCL: o << "      scl_step<MODE>(a, b, " << P << ");\n"; break;
+        case OP_PH:
+          o << "      jph<" << w[1] << ", " << w[2] << ", MODE>(a, b, " << P << ", G);\n";
+          break;
+        case OP_SGN: o << "      jsgn<" << w[1] << "u, MODE>(a, b);\n"; break;
+        case OP_TAB: o << "      tab_step<MODE>(a, b, " << P << ");\n"; break;
+        case OP_THR: o << "      jthr<" << w[1] << ", MODE>(a, b, " << P << ", G);\n"; break;
+        case OP_U1:
+          o << "      ju1<" << w[1] << ", " << w[2] << "u, " << (w[3] ? "true" : "false")
+            << ", MODE>(a, b, " << P << ", G);\n";
+          break;
+        case OP_U2:
+          o << "      ju2<" << w[1] << ", " << w[2] << ", " << (w[3] ? "true" : "false")
+            << ", MODE>(a, b, " << P << ", G);\n";
+          break;
+        case OP_D2: o << "      d2_step<MODE, IDX>(a, b, " << P << ", G);\n"; break;
+        case OP_PROT: {
+          const uint32_t xr = w[1], k = w[6] & 3u;
+          const uint32_t q = xr ? (k + 3u) & 3u : 3u;
+          o << "      jprot<" << xr << ", " << q << ", MODE>(a, b, " << P << ", G);\n";
+          break;
+        }
+        case OP_RED: {
+          const uint32_t xr = w[1];
+          if (mode == MODE_DUAL)
+            o << "      red_cross<" << xr << ", IDX>(a, b, " << P << ", G, acc, " << nred << ");\n";
+          else
+            o << "      red_x<" << xr << ", " << (xr ? RED_PAIR : RED_DIAG) << ", MODE, IDX>(a, b, "
+              << P << ", G, acc, " << nred << ");\n";
+          break;
+        }
+        default: break;
+      }
+      ws += nw;
+    }
+    if (write) o << "      jit_store_slots<MODE>(a, b, tA, tB, G);\n";
+    o << "      __syncthreads();\n    }\n";
+    at = ws;
+  }
+  if (write) o << "    jit_store_tile<" << tb << ", MODE, IDX, TM>(psi, lam, tA, tB, base, tid);\n";
+  o << "  }\n";
+  if (nred) o << "  jit_write_partials<" << NWAVE << ", " << nred << ">(acc, partials, tid, " << NT << ");\n";
+  o << "}\n";
+  return o.str();
+}
+
+bool compile_source(const std::string& src, std::vector<char>& code, std::string& log) {
+  hiprtcProgram p;
+  const char* hdrs[2] = {kJitHdrDevice, kJitHdrProg};
+  const char* names[2] = {"tqsv_device.h", "tqsv_prog.h"};
+  if (hiprtcCreateProgram(&p, src.c_str(), "tqsv_jit_pass.hip", 2, hdrs, names) != HIPRTC_SUCCESS) {
+    log = "hiprtcCreateProgram failed";
+    return false;
+  }
+  // no packed FP32 instructions in device code (as for the precompiled kernels)
+  const char* opts[] = {"--offload-arch=gfx950", "-O3",          "-std=c++17",       "-Xclang",
+                        "-target-feature",       "-Xclang",      "-packed-fp32-ops", "-w"};
+  const hiprtcResult r = hiprtcCompileProgram(p, (int)(sizeof(opts) / sizeof(opts[0])), opts);
+  size_t ls = 0;
+  hiprtcGetProgramLogSize(p, &ls);
+  log.assign(ls, '\0');
+  if (ls) hiprtcGetProgramLog(p, &log[0]);
+  bool ok = r == HIPRTC_SUCCESS;
+  if (ok) {
+    size_t cs = 0;
+    ok = hiprtcGetCodeSize(p, &cs) == HIPRTC_SUCCESS && cs > 0;
+    if (ok) {
+      code.resize(cs);
+      ok = hiprtcGetCode(p, code.data()) == HIPRTC_SUCCESS;
+    }
+  }
+  hiprtcDestroyProgram(&p);
+  return ok;
+}
+
+}  // namespace
+
+int jit_policy(int requested, int n) {
+  if (const char* e = std::getenv("TQSV_JIT")) {
+    if (e[0] == '0') return 0;
+    if (e[0] == '1') return 1;
+  }
+  if (requested > 0) return 1;
+  if (requested < 0) return 0;
+  return n >= 20 ? 1 : 0;
+}
+
+std::string jit_source(int tb, const uint32_t* pass_words, int n) { return gen_source(tb, pass_words, n); }
+
+int jit_prepare(std::vector<PassPlan>& passes, const std::vector<uint32_t>& prog, int n) {
+  int dev = 0;
+  TQSV_HIP(hipGetDevice(&dev));
+  const std::string pre = std::to_string(dev) + "\n";
+  std::vector<std::string> keys(passes.size());
+  std::vector<size_t> todo;  // passes whose kernel is neither cached nor already queued
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    std::unordered_map<std::string, size_t> queued;
+    for (size_t i = 0; i < passes.size(); ++i) {
+      PassPlan& pp = passes[i];
+      pp.jit_fn = nullptr;
+      if (!pp.tile) continue;
+      keys[i] = pre + gen_source(pp.tb, prog.data() + pp.prog_off, n);
+      if (g_cache.count(keys[i]) || queued.count(keys[i])) continue;
+      queued[keys[i]] = i;
+      todo.push_back(i);
+    }
+  }
+  if (!todo.empty()) {
+    std::vector<std::vector<char>> codes(todo.size());
+    std::vector<std::string> logs(todo.size());
+    std::vector<char> ok(todo.size(), 0);
+    std::atomic<size_t> next{0};
+    auto work = [&]() {
+      for (size_t j; (j = next.fetch_add(1)) < todo.size();)
+        ok[j] = compile_source(keys[todo[j]].substr(pre.size()), codes[j], logs[j]) ? 1 : 0;
+    };
+    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    const size_t nthr = std::min<size_t>(hw, todo.size());
+    std::vector<std::thread> th;
+    for (size_t t = 1; t < nthr; ++t) th.emplace_back(work);
+    work();
+    for (auto& t : th) t.join();
+    for (size_t j = 0; j < todo.size(); ++j)
+      if (!ok[j]) return fail(TQSV_EDEVICE, "run-time compilation of a pass kernel failed: " + logs[j]);
+    std::lock_guard<std::mutex> lk(g_mu);
+    for (size_t j = 0; j < todo.size(); ++j) {
+      JitKernel k;
+      TQSV_HIP(hipModuleLoadData(&k.mod, codes[j].data()));
+      TQSV_HIP(hipModuleGetFunction(&k.fn, k.mod, kKernelName));
+      g_cache[keys[todo[j]]] = k;
+    }
+  }
+  std::lock_guard<std::mutex> lk(g_mu);
+  for (size_t i = 0; i < passes.size(); ++i)
+    if (passes[i].tile) passes[i].jit_fn = g_cache[keys[i]].fn;
+  return 0;
+}
+
+hipError_t jit_launch(const PassPlan& pp, const PassLaunch& pl, hipStream_t st) {
+  float2* psi = pl.psi;
+  float2* lam = pl.lam;
+  const uint32_t* prog = pl.prog;
+  double* partials = pl.partials;
+  uint32_t ntiles = pl.ntiles;
+  void* args[] = {&psi, &lam, &prog, &partials, &ntiles};
+  return hipModuleLaunchKernel(static_cast<hipFunction_t>(pp.jit_fn), pl.grid, 1, 1, 1u << pp.tb,
+                               1, 1, 0, st, args, nullptr);
+}
+
+}  // namespace tqsv

@@ -39,11 +39,22 @@ uint32_t mat1_type(const Gate& g) {
   return U1_GENERAL;
 }
 
+// columns orthogonal and of equal norm: a nonzero multiple of a unitary
+bool unitary_like(const cd m[4]) {
+  const double r2 = std::norm(m[0]) + std::norm(m[2]);
+  if (!(r2 > 1e-300) || !std::isfinite(r2)) return false;
+  if (std::abs(std::norm(m[1]) + std::norm(m[3]) - r2) > 1e-9 * r2) return false;
+  return std::abs(std::conj(m[0]) * m[1] + std::conj(m[2]) * m[3]) <= 1e-9 * r2;
+}
+
+// approximate VALU work per amplitude, for the per-pass budget
 int gate_cost(const Gate& g) {
   switch (g.kind) {
-    case GK_MAT1: return (mat1_type(g) == U1_GENERAL ? 8 : 4) + (g.cmask ? 4 : 0);
+    case GK_MAT1:
+      if (g.cmask == 0 && unitary_like(g.m)) return 4;  // rotation + one-sided phase
+      return (mat1_type(g) == U1_GENERAL ? 8 : 4) + (g.cmask ? 4 : 0);
     case GK_MAT2: return g.cmask ? 20 : 16;
-    case GK_DIAG2: return 2;
+    case GK_DIAG2: return 1;
     case GK_PROT: return 8;
     case GK_RED: return 10;
   }
@@ -165,6 +176,10 @@ struct Emit {
     std::memcpy(&v, &x, 4);
     w.push_back(v);
   }
+  void c(cd v) {
+    f((float)v.real());
+    f((float)v.imag());
+  }
   void u64(uint64_t v) {
     u((uint32_t)v);
     u((uint32_t)(v >> 32));
@@ -202,23 +217,360 @@ void block(const Gate& g, uint64_t& blkN, uint64_t& blkD) {
   blkD |= g.db;
 }
 
+// ---- pending diagonal ------------------------------------------------------
+// Diagonal factors are not applied where they occur.  They are kept as the
+// exponent of a quadratic pseudo-Boolean polynomial over index bits,
+//   D(i) = exp(c0 + sum_q lin_q b_q(i) + sum_{p<q} quad_pq b_p(i) b_q(i)),
+// so that the true state is D * (stored state).  A term is applied only when
+// a non-diagonal step is about to act on one of its bits (it commutes with
+// everything applied in between), and whatever remains is applied once at
+// the end of the plan.  Only unit-modulus factors enter lin/quad; c0 also
+// carries the real scale deferred by the rotations.
+constexpr double kTwoPi = 6.283185307179586476925286766559;
+constexpr double kPi = 3.1415926535897932384626433832795;
+
+inline cd wrap(cd v) { return cd(v.real(), std::remainder(v.imag(), kTwoPi)); }
+inline bool trivial(cd v) { return std::abs(v.real()) < 1e-14 && std::abs(v.imag()) < 1e-13; }
+// exp(v) == -1 (a sign)
+inline bool is_sign(cd v) {
+  return std::abs(v.real()) < 1e-14 && std::abs(std::abs(v.imag()) - kPi) < 1e-12;
+}
+inline cd cexp_snap(cd v) {
+  if (trivial(v)) return cd(1, 0);
+  if (is_sign(v)) return cd(-1, 0);
+  return std::exp(v);
+}
+
+struct Pend {
+  cd c0{0, 0};
+  cd lin[64];
+  std::vector<cd> quad;  // [p * 64 + q], p < q
+  uint64_t lmask = 0;
+  uint64_t qm[64] = {};  // qm[p]: bits q with a quad(p, q) term
+  int nrot = 0;          // rotations since the stored state was last rescaled
+  Pend() : quad(64 * 64) {}
+
+  bool empty() const {
+    if (lmask || !trivial(wrap(c0))) return false;
+    for (int p = 0; p < 64; ++p)
+      if (qm[p]) return false;
+    return true;
+  }
+  void add_lin(int q, cd v) {
+    lin[q] = wrap(lin[q] + v);
+    if (trivial(lin[q])) {
+      lin[q] = cd(0, 0);
+      lmask &= ~bitm(q);
+    } else {
+      lmask |= bitm(q);
+    }
+  }
+  cd& qref(int p, int q) { return p < q ? quad[p * 64 + q] : quad[q * 64 + p]; }
+  void add_quad(int p, int q, cd v) {
+    if (p == q) {
+      add_lin(p, v);
+      return;
+    }
+    cd& r = qref(p, q);
+    r = wrap(r + v);
+    if (trivial(r)) {
+      r = cd(0, 0);
+      qm[p] &= ~bitm(q);
+      qm[q] &= ~bitm(p);
+    } else {
+      qm[p] |= bitm(q);
+      qm[q] |= bitm(p);
+    }
+  }
+  void clear_quad(int p, int q) {
+    qref(p, q) = cd(0, 0);
+    qm[p] &= ~bitm(q);
+    qm[q] &= ~bitm(p);
+  }
+  void clear_lin(int q) {
+    lin[q] = cd(0, 0);
+    lmask &= ~bitm(q);
+  }
+  // d[2*bit(a) + bit(b)]; a == b is a one-bit diagonal (d[0], d[3]).
+  // Returns false when a factor is zero or not of unit modulus.
+  bool add_diag2(int a, int b, const cd d[4]) {
+    const int used[4] = {0, a == b ? 3 : 1, 2, 3};
+    cd l[4];
+    for (int j : used) {
+      if (std::abs(std::abs(d[j]) - 1.0) > 1e-12) return false;
+      l[j] = cd(0.0, std::arg(d[j]));
+    }
+    c0 = wrap(c0 + l[0]);
+    if (a == b) {
+      add_lin(a, l[3] - l[0]);
+      return true;
+    }
+    add_lin(a, l[2] - l[0]);
+    add_lin(b, l[1] - l[0]);
+    add_quad(a, b, l[3] - l[2] - l[1] + l[0]);
+    return true;
+  }
+};
+
+// Factorisation U = g k D_L K D_R of a 2x2 matrix proportional to a unitary:
+// D_L = diag(1, eL), D_R = diag(1, eR), K one of the four rotation forms of
+// OP_ROT with parameter t and deferred scale k (cos or sin of the angle).
+struct RotDec {
+  int mode = 0;      // OP_ROT mode
+  double t = 0.0;
+  cd eL{1, 0}, eR{1, 0};
+  cd logk{0, 0};     // log(g k)
+};
+
+inline cd unit(cd v) {
+  const double a = std::abs(v);
+  return a > 0 ? v / a : cd(1, 0);
+}
+inline cd snap_unit(cd v) {
+  if (std::abs(v - cd(1, 0)) < 1e-12) return cd(1, 0);
+  if (std::abs(v + cd(1, 0)) < 1e-12) return cd(-1, 0);
+  if (std::abs(v - cd(0, 1)) < 1e-12) return cd(0, 1);
+  if (std::abs(v + cd(0, 1)) < 1e-12) return cd(0, -1);
+  return v;
+}
+
+bool decompose_u1(const cd m[4], RotDec& d, int hint) {
+  if (!unitary_like(m)) return false;
+  const double r = std::sqrt(std::norm(m[0]) + std::norm(m[2]));
+  const double c = std::abs(m[0]) / r, s = std::abs(m[2]) / r;
+  const cd g = std::abs(m[0]) > 0 ? r * unit(m[0]) : r * unit(m[2]);
+  cd eL, eR;
+  if (s >= c) {
+    eL = unit(m[2] / g);
+    eR = unit(-m[1] / g);
+  } else {
+    eL = std::abs(m[2]) > 0 ? unit(m[2] / g) : cd(1, 0);
+    eR = unit(m[3] / (g * eL));
+  }
+  eL = snap_unit(eL);
+  eR = snap_unit(eR);
+  bool cot = s > c;
+  if (hint == 0 && c > 0 && s <= 4.0 * c) cot = false;
+  if (hint == 1 && s > 0 && c <= 4.0 * s) cot = true;
+  d.t = cot ? c / s : s / c;
+  d.logk = std::log(g * (cot ? s : c));
+  const bool rsign = std::abs(eR.imag()) == 0.0;         // eR = +-1: real form
+  const bool isign = !rsign && std::abs(eR.real()) == 0.0;  // eR = +-i: imaginary form
+  if (isign) {
+    // K_i = [[c, -is], [-is, c]]: eL' = i eL, eR' = -i eR
+    d.mode = cot ? 3 : 2;
+    d.eL = snap_unit(cd(0, 1) * eL);
+    d.eR = snap_unit(cd(0, -1) * eR);
+  } else {
+    d.mode = cot ? 1 : 0;
+    d.eL = eL;
+    d.eR = eR;
+  }
+  return true;
+}
+
+// Slot map of one phase.
+struct Slots {
+  int of_bit[64];
+  uint64_t reg = 0;
+  int bit_of[RB];
+  Slots(const std::array<uint32_t, RB>& Rp, const TileGeo& geo) {
+    for (int b = 0; b < 64; ++b) of_bit[b] = -1;
+    for (int j = 0; j < RB; ++j) bit_of[j] = -1;
+    for (int b = 0; b < 64; ++b) {
+      const int p = geo.pos(b);
+      if (p < 0) continue;
+      for (int j = 0; j < RB; ++j)
+        if ((int)Rp[j] == p) {
+          of_bit[b] = j;
+          bit_of[j] = b;
+          reg |= bitm(b);
+        }
+    }
+  }
+};
+
+struct PhaseEmitter {
+  Pend& P;
+  const Slots& S;
+  Emit& e;
+
+  // apply every pending term whose two bits are both register bits
+  void regreg() {
+    cd tab[NSLOT];
+    for (int s = 0; s < NSLOT; ++s) tab[s] = cd(1, 0);
+    bool any = false;
+    for (uint64_t v = S.reg; v; v &= v - 1) {
+      const int p = ctz(v);
+      for (uint64_t u = P.qm[p] & S.reg & ~lowmask(p + 1); u; u &= u - 1) {
+        const int q = ctz(u);
+        const cd f = cexp_snap(P.qref(p, q));
+        const int kp = S.of_bit[p], kq = S.of_bit[q];
+        for (int s = 0; s < NSLOT; ++s)
+          if (((s >> kp) & 1) && ((s >> kq) & 1)) tab[s] *= f;
+        P.clear_quad(p, q);
+        any = true;
+      }
+    }
+    if (!any) return;
+    bool signs = true;
+    uint32_t mask = 0;
+    for (int s = 0; s < NSLOT; ++s) {
+      if (tab[s] == cd(1, 0)) continue;
+      if (tab[s] == cd(-1, 0)) {
+        mask |= 1u << s;
+        continue;
+      }
+      signs = false;
+    }
+    if (signs) {
+      if (!mask) return;
+      const size_t at = e.begin(OP_SGN);
+      e.u(mask);
+      e.end(at);
+      return;
+    }
+    const size_t at = e.begin(OP_TAB);
+    for (int j = 1; j < TAB_OFF; ++j) e.u(0);
+    for (int s = 0; s < NSLOT; ++s) e.c(tab[s]);
+    e.end(at);
+  }
+
+  // apply every pending term on register bit q
+  void extract(int q) {
+    bool rr = false;
+    for (uint64_t u = P.qm[q]; u; u &= u - 1)
+      if (S.reg & bitm(ctz(u))) rr = true;
+    if (rr) regreg();
+    const cd U0 = cexp_snap(P.lin[q]);
+    std::vector<std::pair<int, cd>> ents;
+    for (uint64_t u = P.qm[q]; u; u &= u - 1) {
+      const int p = ctz(u);
+      ents.push_back({p, cexp_snap(P.qref(q, p))});
+    }
+    for (const auto& x : ents) P.clear_quad(q, x.first);
+    P.clear_lin(q);
+    if (U0 == cd(1, 0) && ents.empty()) return;
+    const size_t at = e.begin(OP_PH);
+    e.u((uint32_t)S.of_bit[q]);
+    e.u((uint32_t)ents.size());
+    e.u(0);
+    e.c(U0);
+    for (const auto& x : ents) {
+      e.u((uint32_t)x.first);
+      e.c(x.second);
+    }
+    e.end(at);
+  }
+
+  // Before a rotation on q: when every pending factor on q is a sign, it
+  // stays pending and flips the rotation instead (Z K Z = K with t -> -t,
+  // times -1 for the cot forms); returns the flip mask and uniform flip.
+  bool frame(int q, uint64_t& zsig, bool& uflip) {
+    bool rr = false;
+    for (uint64_t u = P.qm[q]; u; u &= u - 1)
+      if (S.reg & bitm(ctz(u))) rr = true;
+    if (rr) regreg();
+    zsig = 0;
+    uflip = false;
+    if (P.lmask & bitm(q)) {
+      if (!is_sign(P.lin[q])) return false;
+      uflip = true;
+    }
+    for (uint64_t u = P.qm[q]; u; u &= u - 1) {
+      const int p = ctz(u);
+      if (!is_sign(P.qref(q, p))) return false;
+      zsig |= bitm(p);
+    }
+    return true;
+  }
+
+  // Each rotation grows the stored amplitudes by at most sqrt(1 + t^2) <= 4.2
+  // (|t| <= 4); every 16 rotations the accumulated real scale is applied.
+  // The step is emitted at fixed places, not when the scale gets small, so
+  // that the program structure does not depend on the angles.
+  void rescale() {
+    if (++P.nrot < 16) return;
+    const size_t at = e.begin(OP_SCL);
+    e.f((float)std::exp(P.c0.real()));
+    e.end(at);
+    P.c0 = cd(0, P.c0.imag());
+    P.nrot = 0;
+  }
+
+  void rot(const Gate& g, const RotDec& d) {
+    const int q = g.t0;
+    P.add_lin(q, cd(0, std::arg(d.eR)));
+    uint64_t zsig = 0;
+    bool uflip = false;
+    const bool fr = frame(q, zsig, uflip);
+    if (!fr) {
+      extract(q);
+      zsig = 0;
+      uflip = false;
+    }
+    const size_t at = e.begin(OP_ROT);
+    e.u((uint32_t)S.of_bit[q]);
+    e.u((uint32_t)d.mode);
+    e.f((float)(uflip ? -d.t : d.t));
+    e.u64(zsig);
+    e.end(at);
+    if (d.mode & 1) {
+      // cot forms: Z K Z = -K(-t)
+      if (uflip) P.c0 = wrap(P.c0 + cd(0, kPi));
+      for (uint64_t v = zsig; v; v &= v - 1) P.add_lin(ctz(v), cd(0, kPi));
+    }
+    P.add_lin(q, cd(0, std::arg(d.eL)));
+    P.c0 = wrap(P.c0 + d.logk);
+    rescale();
+  }
+
+  // apply everything that is pending
+  void flush() {
+    regreg();
+    for (int j = 0; j < RB; ++j)
+      if (S.bit_of[j] >= 0) extract(S.bit_of[j]);
+    std::vector<std::array<int, 2>> ab;
+    std::vector<cd> ev;
+    for (uint64_t v = P.lmask; v; v &= v - 1) {
+      const int p = ctz(v);
+      ab.push_back({p, p});
+      ev.push_back(cexp_snap(P.lin[p]));
+      P.clear_lin(p);
+    }
+    for (int p = 0; p < 64; ++p)
+      for (uint64_t u = P.qm[p] & ~lowmask(p + 1); u; u &= u - 1) {
+        const int q = ctz(u);
+        ab.push_back({p, q});
+        ev.push_back(cexp_snap(P.qref(p, q)));
+        P.clear_quad(p, q);
+      }
+    const cd F0 = std::exp(P.c0);
+    P.c0 = cd(0, 0);
+    P.nrot = 0;
+    if (ab.empty() && std::abs(F0 - cd(1, 0)) < 1e-15) return;
+    const size_t at = e.begin(OP_THR);
+    e.u((uint32_t)ab.size());
+    e.c(F0);
+    for (size_t i = 0; i < ab.size(); ++i) {
+      e.u((uint32_t)ab[i][0]);
+      e.u((uint32_t)ab[i][1]);
+      e.c(ev[i]);
+    }
+    e.end(at);
+  }
+};
+
 // emit one phase's steps
 void emit_phase(std::vector<Gate>& gates, const std::vector<int>& sel,
                 const std::array<uint32_t, RB>& Rp, const TileGeo& geo, Emit& e, uint32_t& nred,
-                uint32_t red_base) {
-  auto slot_of_pos = [&](int p) -> int {
-    for (int j = 0; j < RB; ++j)
-      if ((int)Rp[j] == p) return j;
-    return -1;
-  };
-  auto slot_of_bit = [&](int b) -> int {
-    const int p = geo.pos(b);
-    return p < 0 ? -1 : slot_of_pos(p);
-  };
+                uint32_t red_base, Pend& P, bool flush_at_end) {
+  const Slots S(Rp, geo);
+  PhaseEmitter pe{P, S, e};
   auto slotmask = [&](uint64_t bits) -> uint32_t {
     uint32_t m = 0;
     for (uint64_t v = bits; v; v &= v - 1) {
-      const int s = slot_of_bit(ctz(v));
+      const int s = S.of_bit[ctz(v)];
       if (s >= 0) m |= 1u << s;
     }
     return m;
@@ -230,71 +582,34 @@ void emit_phase(std::vector<Gate>& gates, const std::vector<int>& sel,
       if (popc((uint64_t)(s & zs)) & 1) m |= 1u << s;
     return m;
   };
-
-  cd tab[NSLOT];
-  bool tab_used = false;
-  uint32_t tab_deps = 0;
-  struct Ent {
-    int a, b;
-    cd d[4];
-  };
-  std::vector<Ent> ents;
-  size_t cz_at = (size_t)-1;  // open OP_CZ step (entries may be appended)
-  auto reset_tab = [&]() {
-    for (int s = 0; s < NSLOT; ++s) tab[s] = cd(1, 0);
-    tab_used = false;
-    tab_deps = 0;
-    ents.clear();
-  };
-  reset_tab();
-  auto flush = [&]() {
-    if (!tab_used && ents.empty()) return;
-    cz_at = (size_t)-1;
-    const size_t at = e.begin(OP_DIAG);
-    e.u(tab_used ? 1u : 0u);
-    e.u((uint32_t)ents.size());
-    e.u(0);
-    if (tab_used)
-      for (int s = 0; s < NSLOT; ++s) {
-        e.f((float)tab[s].real());
-        e.f((float)tab[s].imag());
-      }
-    for (const Ent& x : ents) {
-      e.u((uint32_t)x.a);
-      e.u((uint32_t)x.b);
-      for (int j = 0; j < 4; ++j) {
-        e.f((float)x.d[j].real());
-        e.f((float)x.d[j].imag());
-      }
-    }
-    e.end(at);
-    reset_tab();
+  auto extract_bits = [&](uint64_t bits) {
+    for (uint64_t v = bits; v; v &= v - 1) pe.extract(ctz(v));
   };
 
   for (int gi : sel) {
     Gate& g = gates[gi];
     switch (g.kind) {
       case GK_MAT1: {
-        const int k = slot_of_bit(g.t0);
-        if (tab_deps & (1u << k)) flush();
-        cz_at = (size_t)-1;
+        RotDec d;
+        if (g.cmask == 0 && decompose_u1(g.m, d, g.rot_form)) {
+          g.rot_form = d.mode & 1;
+          pe.rot(g, d);
+          break;
+        }
+        extract_bits(bitm(g.t0));
         const size_t at = e.begin(OP_U1);
-        e.u((uint32_t)k);
+        e.u((uint32_t)S.of_bit[g.t0]);
         e.u(mat1_type(g));
         e.u(g.cmask ? 1u : 0u);
         e.u64(g.cmask);
         e.u64(g.cval);
-        for (int j = 0; j < 4; ++j) {
-          e.f((float)g.m[j].real());
-          e.f((float)g.m[j].imag());
-        }
+        for (int j = 0; j < 4; ++j) e.c(g.m[j]);
         e.end(at);
         break;
       }
       case GK_MAT2: {
-        int k0 = slot_of_bit(g.t0), k1 = slot_of_bit(g.t1);
-        if (tab_deps & ((1u << k0) | (1u << k1))) flush();
-        cz_at = (size_t)-1;
+        extract_bits(bitm(g.t0) | bitm(g.t1));
+        int k0 = S.of_bit[g.t0], k1 = S.of_bit[g.t1];
         cd mm[16];
         if (k0 > k1) {
           for (int j = 0; j < 16; ++j) mm[j] = g.m[j];
@@ -310,60 +625,24 @@ void emit_phase(std::vector<Gate>& gates, const std::vector<int>& sel,
         e.u(g.cmask ? 1u : 0u);
         e.u64(g.cmask);
         e.u64(g.cval);
-        for (int j = 0; j < 16; ++j) {
-          e.f((float)mm[j].real());
-          e.f((float)mm[j].imag());
-        }
+        for (int j = 0; j < 16; ++j) e.c(mm[j]);
         e.end(at);
         break;
       }
       case GK_DIAG2: {
-        const int ka = slot_of_bit(g.t0), kb = slot_of_bit(g.t1);
-        if (ka >= 0 && kb >= 0) {
-          for (int s = 0; s < NSLOT; ++s) {
-            const int ba = (s >> ka) & 1, bb = (s >> kb) & 1;
-            tab[s] *= g.m[2 * ba + bb];
-          }
-          tab_used = true;
-          tab_deps |= (1u << ka) | (1u << kb);
-        } else if (ka < 0 && kb < 0) {
-          Ent x;
-          x.a = g.t0;
-          x.b = g.t1;
-          for (int j = 0; j < 4; ++j) x.d[j] = g.m[j];
-          ents.push_back(x);
-        } else if (is1(g.m[0]) && is1(g.m[1]) && is1(g.m[2]) && g.m[3] == cd(-1, 0)) {
-          const int lo = std::min(g.t0, g.t1), hi = std::max(g.t0, g.t1);
-          if (cz_at == (size_t)-1) {
-            cz_at = e.begin(OP_CZ);
-            e.u(0);
-            e.end(cz_at);
-          }
-          // append an entry to the open CZ step (it is the last step emitted)
-          e.w[cz_at + 1] += 1;
-          e.u64(bitm(lo));
-          e.u((uint32_t)(hi - lo));
-          e.w[cz_at] = (e.w[cz_at] & 0xffffu) | (uint32_t)((e.w.size() - cz_at) << 16);
-        } else {
-          cz_at = (size_t)-1;
-          const size_t at = e.begin(OP_D2);
-          e.u((uint32_t)g.t0);
-          e.u((uint32_t)g.t1);
-          e.u(0);
-          for (int j = 0; j < 4; ++j) {
-            e.f((float)g.m[j].real());
-            e.f((float)g.m[j].imag());
-          }
-          e.end(at);
-        }
+        if (P.add_diag2(g.t0, g.t1, g.m)) break;
+        const size_t at = e.begin(OP_D2);
+        e.u((uint32_t)g.t0);
+        e.u((uint32_t)g.t1);
+        e.u(0);
+        for (int j = 0; j < 4; ++j) e.c(g.m[j]);
+        e.end(at);
         break;
       }
       case GK_PROT: {
-        const uint32_t xr = slotmask(g.x);
-        if (tab_deps & xr) flush();
-        cz_at = (size_t)-1;
+        extract_bits(g.x);
         const size_t at = e.begin(OP_PROT);
-        e.u(xr);
+        e.u(slotmask(g.x));
         e.u64(g.z);
         e.f((float)g.c);
         e.f((float)g.s);
@@ -372,8 +651,10 @@ void emit_phase(std::vector<Gate>& gates, const std::vector<int>& sel,
         break;
       }
       case GK_RED: {
-        flush();
-        cz_at = (size_t)-1;
+        // <lambda|P|psi> with both states carrying the same pending factor D:
+        // the terms on the flip bits are applied first; the rest cancels
+        // except |exp(c0)|^2, which the host applies to the result.
+        extract_bits(g.x);
         const uint32_t xr = slotmask(g.x);
         const uint32_t zs = slotmask(g.z);
         const uint32_t sm = smask_of(zs);
@@ -389,12 +670,13 @@ void emit_phase(std::vector<Gate>& gates, const std::vector<int>& sel,
         }
         e.end(at);
         g.red_slot = (int)(red_base + nred);
+        g.red_scale = std::exp(2.0 * P.c0.real());
         nred += 2;
         break;
       }
     }
   }
-  flush();
+  if (flush_at_end) pe.flush();
 }
 
 }  // namespace
@@ -537,6 +819,7 @@ int schedule(std::vector<Gate>& gates, const SchedOpts& so, std::vector<PassPlan
   const int T = effective_tile_bits(n, so.tile_bits, so.dual);
   const bool tiles_ok = so.fuse && T >= 11 && !(so.dual && n > 32);
   const int TBv = T - RB;
+  const int mode = so.dual ? MODE_DUAL : MODE_GATE;
   int Lmin = so.min_low > 0 ? so.min_low : 5;
   if (Lmin > T - RB) Lmin = T - RB;
   if (Lmin < T - MAXH) Lmin = T - MAXH;  // at most MAXH high bits fit the pass header
@@ -547,20 +830,19 @@ int schedule(std::vector<Gate>& gates, const SchedOpts& so, std::vector<PassPlan
   std::vector<int> rem(gates.size());
   for (size_t i = 0; i < gates.size(); ++i) rem[i] = (int)i;
   std::vector<char> done(gates.size(), 0);
+  std::vector<PassPlan> plan;
+  std::vector<TileGeo> geos;
 
+  // ---- pass and phase selection
   while (!rem.empty()) {
     PassPlan pp;
     pp.dual = so.dual;
+    pp.mode = mode;
     if (!tiles_ok || !fusable(gates[rem[0]])) {
       pp.tile = false;
       pp.single = rem[0];
-      if (gates[rem[0]].kind == GK_RED) {
-        pp.nred = 2;
-        pp.red_base = red_total;
-        gates[rem[0]].red_slot = (int)red_total;
-        red_total += 2;
-      }
-      passes.push_back(pp);
+      plan.push_back(pp);
+      geos.emplace_back();
       rem.erase(rem.begin());
       continue;
     }
@@ -665,12 +947,35 @@ int schedule(std::vector<Gate>& gates, const SchedOpts& so, std::vector<PassPlan
       if (!done[gi]) nrem.push_back(gi);
     rem.swap(nrem);
 
-    // 4) emit the pass record
     pp.tile = true;
     pp.tb = TBv;
     pp.L = (uint32_t)geo.L;
     pp.m = (uint32_t)geo.H.size();
     for (size_t k = 0; k < geo.H.size(); ++k) pp.H[k] = (uint32_t)geo.H[k];
+    plan.push_back(pp);
+    geos.push_back(geo);
+  }
+
+  // ---- emission.  The pending diagonal travels from pass to pass; it is
+  // flushed in the last phase of the last tile pass before a single-op pass
+  // (which does not know about it) and at the end of the plan.
+  Pend P;
+  for (size_t i = 0; i < plan.size(); ++i) {
+    PassPlan& pp = plan[i];
+    if (!pp.tile) {
+      Gate& g = gates[pp.single];
+      if (g.kind == GK_RED) {
+        pp.nred = 2;
+        pp.red_base = red_total;
+        g.red_slot = (int)red_total;
+        g.red_scale = 1.0;
+        red_total += 2;
+      }
+      passes.push_back(pp);
+      continue;
+    }
+    const bool flush = i + 1 == plan.size() || !plan[i + 1].tile;
+    const TileGeo& geo = geos[i];
     pp.prog_off = prog.size();
     pp.red_base = red_total;
     Emit e(prog);
@@ -680,8 +985,8 @@ int schedule(std::vector<Gate>& gates, const SchedOpts& so, std::vector<PassPlan
       const size_t hdr = prog.size();
       for (int k = 0; k < PS_SIZE; ++k) e.u(0);
       const size_t steps0 = prog.size();
-      emit_phase(gates, pp.phases[ip], pp.Rpos[ip], geo, e, nred, red_total);
-      // count steps
+      emit_phase(gates, pp.phases[ip], pp.Rpos[ip], geo, e, nred, red_total, P,
+                 flush && ip + 1 == pp.phases.size());
       uint32_t nsteps = 0;
       for (size_t at = steps0; at < prog.size(); at += prog[at] >> 16) ++nsteps;
       prog[hdr + PS_NSTEPS] = nsteps;
@@ -701,10 +1006,13 @@ int schedule(std::vector<Gate>& gates, const SchedOpts& so, std::vector<PassPlan
     pp.ntiles = (uint32_t)(1ull << (n - T));
     const uint32_t cap = nred ? 2048u : 65536u;
     pp.grid = pp.ntiles < cap ? pp.ntiles : cap;
-    pp.lds = pass_lds_bytes(TBv, so.dual, nred);
+    pp.lds = pass_lds_bytes(TBv, mode, nred);
     if (pp.lds > 160 * 1024) return fail(TQSV_EUNSUPPORTED, "pass needs more LDS than available");
+    if (!pass_variant_exists(TBv, mode, n > 32))
+      return fail(TQSV_EUNSUPPORTED, "no tile-pass kernel variant for this tile size");
     passes.push_back(pp);
   }
+  if (!P.empty()) return fail(TQSV_EINVAL, "planner left diagonal factors unapplied (internal error)");
   return 0;
 }
 
@@ -759,7 +1067,10 @@ int run_passes(const std::vector<PassPlan>& passes, const std::vector<Gate>& gat
       pl.ntiles = pp.ntiles;
       pl.grid = pp.grid;
       pl.lds_bytes = pp.lds;
-      TQSV_HIP(launch_pass(pp.tb, pp.dual, n > 32, pl, st));
+      if (pp.jit_fn)
+        TQSV_HIP(jit_launch(pp, pl, st));
+      else
+        TQSV_HIP(launch_pass(pp.tb, pp.mode, n > 32, pl, st));
       if (pp.nred) TQSV_HIP(k_final_sum(dpartials, pp.grid, pp.nred, dred_out + pp.red_base, st));
     } else {
       const Gate& g = gates[pp.single];

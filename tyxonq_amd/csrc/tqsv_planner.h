@@ -5,6 +5,7 @@
 #include <array>
 #include <complex>
 #include <cstdint>
+#include <string>
 #include <vector>
 
 #include "../../include/tqsv.h"
@@ -39,6 +40,11 @@ struct Gate {
   uint64_t gx = 0, gz = 0;
   double gcoef = 0.0;
   int red_slot = -1;
+  double red_scale = 1.0;  // true <lambda|P|psi> = red_scale * (value read from the sweep)
+  // rotation form (tan 0 / cot 1) chosen when the gate was last emitted; kept
+  // across parameter updates while |t| <= 4 so that the program structure (and
+  // any compiled kernel) does not flip when an angle moves across |t| = 1
+  int rot_form = -1;
   int src = -1;
 };
 
@@ -53,6 +59,7 @@ struct PassPlan {
   bool tile = false;     // tile pass; else a single-op launch of `single`
   int tb = 0;
   bool dual = false;
+  int mode = 0;          // kernel mode (MODE_GATE / MODE_DUAL)
   uint32_t L = 0, m = 0;
   uint32_t H[MAXH] = {};
   std::vector<std::vector<int>> phases;   // gate indices per phase
@@ -65,6 +72,7 @@ struct PassPlan {
   size_t prog_words = 0;
   uint32_t ntiles = 0, grid = 0;
   size_t lds = 0;
+  void* jit_fn = nullptr;  // generated kernel (hipFunction_t), when the plan uses them
 };
 
 struct SchedOpts {
@@ -92,11 +100,23 @@ int run_passes(const std::vector<PassPlan>& passes, const std::vector<Gate>& gat
 
 int effective_tile_bits(int n, int requested, bool dual);
 
+// ---- generated pass kernels (tqsv_jit.cpp) ----
+// 1 if plans of n qubits use generated kernels (requested: 1 on, -1 off, 0
+// automatic; the environment variable TQSV_JIT=0/1 overrides).
+int jit_policy(int requested, int n);
+// Compile (or take from the process-wide cache) the kernel of every tile pass
+// and set PassPlan::jit_fn.
+int jit_prepare(std::vector<PassPlan>& passes, const std::vector<uint32_t>& prog, int n);
+hipError_t jit_launch(const PassPlan& pp, const PassLaunch& pl, hipStream_t st);
+// generated source of one pass record (for inspection and tests)
+std::string jit_source(int tb, const uint32_t* pass_words, int n);
+
 }  // namespace tqsv
 
 struct tqsv_plan {
   int n = 0;
   int nops = 0;
+  int jit = 0;  // passes run as generated kernels
   tqsv::SchedOpts so;
   std::vector<tqsv::Gate> gates;          // forward gates after merge
   std::vector<tqsv::PassPlan> passes;

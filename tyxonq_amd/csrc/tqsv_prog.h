@@ -1,5 +1,6 @@
 // tqsv_prog.h - word layout of the tile-pass programs that the host planner
-// (tqsv_plan.cpp) writes and the pass kernel (tqsv_pass.hip) interprets.
+// (tqsv_plan.cpp, tqsv_ham.cpp) writes and the pass kernel (tqsv_pass_impl.h)
+// interprets.
 //
 // A state of 2^n amplitudes is swept in tiles of 2^T amplitudes.  A tile is
 // the set of amplitudes that agree on every index bit outside the tile's bit
@@ -7,11 +8,19 @@
 // the global index bits as: p bits [0, L) -> index bits [0, L); p bit L+k ->
 // index bit H[k].  A work-group holds one tile in LDS and runs the pass's
 // phases over it; in a phase every thread holds 2^RB amplitudes in registers
-// whose tile positions differ only in the RB "register positions" R[0..RB).
+// ("slots") whose tile positions differ only in the RB "register positions"
+// R[0..RB).  Slot s holds the amplitude whose register-position bits equal the
+// bits of s; the other index bits ("thread bits") are fixed per thread and
+// are read from the thread's global index gb (the index of its slot 0).
 #ifndef TQSV_PROG_H
 #define TQSV_PROG_H
 
+#ifdef __HIPCC_RTC__
+typedef unsigned int uint32_t;
+typedef unsigned long long uint64_t;
+#else
 #include <stdint.h>
+#endif
 
 namespace tqsv {
 
@@ -22,6 +31,11 @@ constexpr int MAXH = 9;            // maximum number of non-contiguous tile bits
 // pass flags
 constexpr uint32_t PF_WRITE = 1u;  // write the tile back (gate pass); else read-only
 constexpr uint32_t PF_DUAL = 2u;   // a second state (lambda) travels with psi
+
+// kernel modes (one kernel variant per mode)
+constexpr int MODE_GATE = 0;  // psi only, gate steps, written back
+constexpr int MODE_RED = 1;   // psi only, reduction steps, read-only
+constexpr int MODE_DUAL = 2;  // psi and lambda, gate steps and cross reductions
 
 // pass header (word offsets)
 constexpr int PH_FLAGS = 0;
@@ -38,36 +52,51 @@ constexpr int PS_NSTEPS = 0;
 constexpr int PS_R = 1;           // R[0..RB), ascending
 constexpr int PS_SIZE = 8;
 
-// step word 0: opcode | (nwords << 16); all offsets below are from the step start.
+// Step word 0: opcode | (nwords << 16); all offsets below are from the step
+// start.  "k" is a slot bit (0..RB-1); "g" is a global index bit.
 enum : uint32_t {
-  // 2x2 matrix on register slot bit k.
+  // general 2x2 matrix on slot bit k, optionally controlled.
   // [1]=k [2]=type [3]=ctrl(0/1) [4..6)=cmask(lo,hi) [6..8)=cval(lo,hi) [8..16)=m (floats)
   OP_U1 = 1,
-  // 4x4 matrix on register slot bits (kh, kl), kh > kl, kh the MSB of the basis.
+  // 4x4 matrix on slot bits (kh, kl), kh > kl, kh the MSB of the basis.
   // [1]=kh [2]=kl [3]=ctrl [4..6)=cmask [6..8)=cval [8..40)=m (floats)
   OP_U2 = 2,
-  // diagonal: amp[s] *= tab[s] * prod_e d_e[2*bit(ga_e)+bit(gb_e)] where the
-  // product runs over entries whose bits are outside the register positions.
-  // [1]=ntab(0/1) [2]=nthr [3]=pad [4..68)=tab (32 complex, when ntab)
-  // then nthr entries of {ga, gb, d (8 floats)} = 10 words
-  OP_DIAG = 3,
-  // sign flip by parity of popcount(i & (i >> shift) & mask), for batches of CZ.
-  // [1]=cnt then cnt entries {mask lo, mask hi, shift} = 3 words
-  OP_CZ = 4,
+  // rotation with deferred scale on slot bit k (x0 = a[s], x1 = a[s | 1<<k]):
+  //   mode 0: y0 = x0 - t x1,   y1 = x1 + t x0
+  //   mode 1: y0 = t x0 - x1,   y1 = t x1 + x0
+  //   mode 2: y0 = x0 - i t x1, y1 = x1 - i t x0
+  //   mode 3: y0 = t x0 - i x1, y1 = t x1 - i x0
+  // t is negated in threads where parity(gb & zsig) = 1.
+  // [1]=k [2]=mode [3]=t (float) [4..6)=zsig (lo,hi)
+  OP_ROT = 3,
+  // one-sided phase on slot bit k: a[s] *= u for slots with bit k set, where
+  // u = U0 * prod_{entries e with bit g_e of gb set} E_e.
+  // [1]=k [2]=nent [3]=pad [4..6)=U0 (re,im), then nent entries {g, Er, Ei}
+  OP_PH = 4,
+  // sign flips: a[s] = -a[s] for every slot s whose bit is set in the mask.  [1]=mask
+  OP_SGN = 5,
+  // uniform slot table: a[s] *= tab[s].  [1..4)=pad [4..68)=tab (32 complex)
+  OP_TAB = 6,
+  // per-thread complex factor: a[s] *= F for all s, F = F0 * prod_{entries e with bits
+  // ga_e and gb_e of gb both set} E_e.
+  // [1]=nent [2..4)=F0 (re,im), then nent entries {ga, gb, Er, Ei}
+  OP_THR = 7,
   // 2-bit diagonal on arbitrary index bits: amp *= d[2*bit(ga)+bit(gb)]
   // [1]=ga [2]=gb [3]=pad [4..12)=d (4 complex)
-  OP_D2 = 5,
+  OP_D2 = 8,
   // Pauli rotation exp(-i theta/2 P), flip bits inside the register slots.
   // [1]=xr (slot mask) [2..4)=z (lo,hi) [4]=cos(theta/2) [5]=sin(theta/2)
   // [6]=k with omega = i^k, k=|x&z| mod 4
-  OP_PROT = 6,
+  OP_PROT = 9,
   // reduction group.  [1]=xr [2]=kind [3]=nterms, then nterms entries
   // {slot, zt lo, zt hi, smask, part} = 5 words.
   //  kind 0: pairs s (lowest set bit of xr clear): v = conj(a[s^xr]) a[s]
   //  kind 1: all s: v = |a[s]|^2 (xr = 0)
-  //  kind 2: all s: v = conj(b[s]) a[s^xr]  (b = lambda, DUAL passes)
-  // term value += sum_s (-1)^{smask bit s} (-1)^{|gbase & zt|} part(v)
-  OP_RED = 7,
+  //  kind 2: all s: v = conj(b[s]) a[s^xr]  (b = lambda, dual passes)
+  // term value += sum_s (-1)^{smask bit s} (-1)^{|gb & zt|} part(v)
+  OP_RED = 10,
+  // real scale: a[s] *= f for all s.  [1]=f (float)
+  OP_SCL = 11,
 };
 
 // OP_U1 types
@@ -80,9 +109,11 @@ constexpr uint32_t RED_PAIR = 0;
 constexpr uint32_t RED_DIAG = 1;
 constexpr uint32_t RED_CROSS = 2;
 
-constexpr int DIAG_HDR = 4;
-constexpr int DIAG_TAB = 64;
-constexpr int DIAG_ENT = 10;
+constexpr int OPH_HDR = 6;
+constexpr int OPH_ENT = 3;
+constexpr int THR_HDR = 4;
+constexpr int THR_ENT = 4;
+constexpr int TAB_OFF = 4;
 constexpr int RED_HDR = 4;
 constexpr int RED_ENT = 5;
 
