@@ -1,19 +1,27 @@
 #!/usr/bin/env python3
 """Benchmark of the MI355X statevector engine (one GPU).
 
-Default workload: 30-qubit complex64 brickwork of 20 layers (RX.RZ on every
-qubit, CZ on even/odd pairs) from |0...0>, then the norm and the energy of a
-random 500-term 2-local Pauli sum, both reduced in FP64.  One step = that
-whole job with the state resident in HBM.
+Default workload (--workload brickwork): the 30-qubit complex64 brickwork of
+20 layers (RX.RZ on every qubit, CZ on even/odd pairs) from |0...0>, then the
+norm and the energy of a random 500-term 2-local Pauli sum, both reduced in
+FP64.  One step = that whole job with the state resident in HBM.
 
-Output: one JSON line.  `value` is the step's algorithmic HBM traffic over the
-step time (GB/s): one state write for the initial state (8 B/amp), every gate
-pass reads and writes the state (16 B/amp), every expectation pass reads it
-(8 B/amp); passes are counted from the launch list.  `roofline` describes the
-dominant kernel (the fused gate pass) with its own launch-averaged duration
-measured by HIP events on the launch stream; `cpu_baseline` times the CPU
-oracle (the reference's per-gate contraction restated in NumPy complex128)
-on a bounded sample of the same 30-qubit workload.
+`value` (GB/s) = the step's layer-equivalent HBM traffic over the step time:
+the initial state write (8 B/amp), one read and one write of the state per
+gate layer (16 B/amp per layer, what a layer-by-layer engine has to move) and
+one read for the expectation (8 B/amp).  Fused passes move less than that, so
+value / 8000 is the "fraction of HBM peak per gate layer" of the target.
+`roofline` is the measured per-launch bandwidth of the dominant kernel (the
+fused gate pass: one read and one write of the state per launch) from HIP
+events on its launch stream; `traffic` comes from the committed rocprofv3
+FETCH_SIZE / WRITE_SIZE summary when one exists for this workload.
+`cpu_baseline` times the oracle (NumPy complex128, the reference's per-gate
+contraction) on a bounded sample of the same workload on the host cores.
+
+Other workloads: --workload adjoint (22-qubit RY.RZ + CX-ring ansatz, 528
+parameters, periodic Heisenberg chain: adjoint energy+gradient vs the plain
+per-parameter GPU path), --workload hea (12-qubit hardware-efficient ansatz,
+open-chain TFIM energy+gradient).
 
     python bench.py [--steps K] [--warmup W] [--workload brickwork|adjoint|hea]
 """
@@ -28,6 +36,8 @@ import time
 import numpy as np
 
 HBM_PEAK_GBPS = 8000.0
+METRIC = "30q gate GB/s (% HBM peak); VQE energy+grad/s and p99 latency vs plain"
+ROOT = os.path.dirname(os.path.abspath(__file__))
 
 
 def parse():
@@ -41,12 +51,14 @@ def parse():
     ap.add_argument("--terms", type=int, default=500)
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--tile-bits", type=int, default=0)
-    ap.add_argument("--max-cost", type=int, default=0)
+    ap.add_argument("--max-cost", type=int, default=200)
+    ap.add_argument("--jit", type=int, default=0, help="1 generated kernels, -1 precompiled, 0 auto")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calib-copy", action="store_true",
-                    help="also run one device copy of the state (HBM counter calibration)")
-    ap.add_argument("--check", action="store_true",
-                    help="compare the energy with a fused-vs-unfused run")
+                    help="after the timed steps, copy the state once with a device copy "
+                         "kernel of known size (calibration of HBM byte counters)")
+    ap.add_argument("--shift-sample", type=int, default=8,
+                    help="adjoint workload: parameters timed on the parameter-shift path")
     return ap.parse_args()
 
 
@@ -62,42 +74,54 @@ def _events():
     return torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
 
 
-def cpu_baseline_brickwork(n: int, seed: int) -> dict:
-    """Time the oracle's per-gate contraction (complex128) on three gates of
-    the 30-qubit brickwork: RX on qubit 0, RZ on qubit n//2, CZ(n//2, n//2+1)."""
-    from oracle import statevector as O
-
+def _threads() -> int:
     try:
         from threadpoolctl import threadpool_info
 
-        threads = max([d.get("num_threads", 1) for d in threadpool_info()] or [1])
+        return int(max([d.get("num_threads", 1) for d in threadpool_info()] or [1]))
     except Exception:
-        threads = 1
+        return 1
+
+
+def _traffic(name: str):
+    """HBM bytes per launch from a committed rocprofv3 counter summary."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{name}.json")
+    try:
+        with open(path) as f:
+            return json.load(f).get("bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+# ---- config 2: 30-qubit brickwork + 500-term <H> ----------------------------------
+def cpu_baseline_brickwork(n: int, layers: int) -> dict:
+    """The oracle's per-gate contraction (complex128) on two gates of the
+    workload: RX on the middle qubit and CZ on the middle pair; a layer of the
+    reference's dispatch is 2n one-qubit gates and about n/2 CZs."""
+    from oracle import statevector as O
+
     psi = np.zeros(1 << n, dtype=np.complex128)
     psi[0] = 1.0
-    samples = []
+    q = n // 2
     t0 = time.perf_counter()
-    psi = O.apply_1q(psi, O.gate_rx(0.3), 0, n)
-    samples.append(time.perf_counter() - t0)
+    psi = O.apply_1q(psi, O.gate_rx(0.3), q, n)
+    t1q = time.perf_counter() - t0
     t0 = time.perf_counter()
-    psi = O.apply_1q(psi, O.gate_rz(0.7), n // 2, n)
-    samples.append(time.perf_counter() - t0)
-    t0 = time.perf_counter()
-    psi = O.apply_2q(psi, O.gate_cz(), n // 2, n // 2 + 1, n)
-    samples.append(time.perf_counter() - t0)
+    psi = O.apply_2q(psi, O.gate_cz(), q, q + 1, n)
+    t2q = time.perf_counter() - t0
     del psi
-    per_gate = float(np.mean(samples))
-    gbps = 2 * 16 * (1 << n) / per_gate / 1e9
+    layer_s = 2 * n * t1q + (n // 2) * t2q
+    job_s = layers * layer_s
     return {
-        "value": round(gbps, 3),
+        "value": round(layers * 16 * (1 << n) / job_s / 1e9, 3),
         "unit": "GB/s",
-        "cores": int(threads),
+        "cores": _threads(),
         "kind": "port",
-        "sample": (f"NumPy complex128 per-gate contraction (oracle restatement of the reference's "
-                   f"apply_1q/apply_2q) on 3 gates of the {n}-qubit brickwork (RX q0, RZ q{n//2}, "
-                   f"CZ q{n//2},{n//2+1}); value = 2*16*2^n B per gate / mean gate time"),
-        "per_gate_s": round(per_gate, 3),
-        "job_s_extrapolated_gates_only": round(per_gate * (20 * (2 * n) + 290), 1),
+        "sample": (f"NumPy complex128 per-gate contraction (the oracle's restatement of the "
+                   f"reference's apply_1q/apply_2q) timed on 2 gates of the {n}-qubit workload "
+                   f"(RX q{q}: {t1q:.2f} s, CZ q{q},{q + 1}: {t2q:.2f} s), extrapolated to "
+                   f"{layers} layers of {2 * n} one-qubit gates + {n // 2} CZs "
+                   f"({job_s:.0f} s); value counts 16 B/amp per layer like the GPU value"),
     }
 
 
@@ -115,25 +139,20 @@ def run_brickwork(args) -> dict:
     terms = W.random_2local_terms(n, args.terms, args.seed)
     L, _, _ = lower(circ)
     ops = L.segments[0][1]
-    plan = nat.Plan(n, ops, tile_bits=args.tile_bits, max_cost=args.max_cost)
-    info = plan.info()
+    t0 = time.perf_counter()
+    plan = nat.Plan(n, ops, tile_bits=args.tile_bits, max_cost=args.max_cost, jit=args.jit)
     x, z, c = PauliSum(n, terms).to_masks()
     ham = nat.Ham(n, x, z, c)
-    dev = nat.device()
-    psi = torch.empty(1 << n, dtype=torch.complex64, device=dev)
-    n_exp_passes = None
-
-    def step():
-        nat.init_basis(psi, 0)
-        plan.run(psi)
-        return ham.expval(psi)
+    build_s = time.perf_counter() - t0
+    info = plan.info()
+    psi = torch.empty(1 << n, dtype=torch.complex64, device=nat.device())
 
     for _ in range(args.warmup):
-        step()
+        nat.init_basis(psi, 0)
+        plan.run(psi)
+        ham.expval(psi)
     _sync()
-    # expectation pass count from the plan of the Hamiltonian: measured by timing below
-    gate_ms, exp_ms = [], []
-    energies = []
+    gate_ms, exp_ms, energies = [], [], []
     t_start = time.perf_counter()
     for _ in range(args.steps):
         nat.init_basis(psi, 0)
@@ -141,27 +160,27 @@ def run_brickwork(args) -> dict:
         e0.record()
         plan.run(psi)
         e1.record()
-        t0 = time.perf_counter()
+        e2, e3 = _events()
+        e2.record()
         energies.append(ham.expval(psi))  # synchronizes the stream
-        exp_ms.append((time.perf_counter() - t0) * 1e3)
+        e3.record()
+        e3.synchronize()
+        exp_ms.append(e2.elapsed_time(e3))
         gate_ms.append(e0.elapsed_time(e1))
     _sync()
-    t_total = time.perf_counter() - t_start
-    ms_per_step = t_total / args.steps * 1e3
+    ms_per_step = (time.perf_counter() - t_start) / args.steps * 1e3
     dim = 1 << n
-    gate_pass_bytes = 16 * dim
     passes = info["passes"]
-    n_exp_passes = ham_passes(ham)
-    step_bytes = 8 * dim + passes * gate_pass_bytes + n_exp_passes * 8 * dim
+    step_bytes = 8 * dim + layers * 16 * dim + 8 * dim
     value = step_bytes / (ms_per_step / 1e3) / 1e9
     avg_gate_ms = float(np.mean(gate_ms))
     per_launch_ms = avg_gate_ms / passes
-    achieved = gate_pass_bytes / (per_launch_ms / 1e3) / 1e9
+    achieved = 16 * dim / (per_launch_ms / 1e3) / 1e9
     energy, norm = energies[-1]
-    ngates = len(ops)
+    workload = f"brickwork {n}q x {layers} layers + {args.terms}-term 2-local <H>"
     out = {
-        "metric": "30q gate GB/s (% HBM peak); VQE energy+grad/s and p99 latency vs plain",
-        "value": round(value, 2),
+        "metric": METRIC,
+        "value": round(value, 1),
         "unit": "GB/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -170,18 +189,21 @@ def run_brickwork(args) -> dict:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "c64 (fp32 re/im), fp64 reductions",
+        "dtype": "c64",
         "data": "synthetic (seeded brickwork circuit and random 2-local Pauli sum)",
-        "config": {"workload": f"brickwork {n}q x {layers} layers + {args.terms}-term 2-local <H>",
-                   "qubits": n, "layers": layers, "gates": ngates, "pauli_terms": args.terms,
-                   "seed": args.seed, "gate_passes": passes, "expval_passes": n_exp_passes,
-                   "tile_bits": info["tile_bits"], "phases": info["phases"],
-                   "gates_after_merge": info["gates_after_merge"], "parallelism": "1 GPU"},
+        "config": {"workload": workload, "qubits": n, "layers": layers,
+                   "gates": int(len(ops)), "pauli_terms": args.terms, "seed": args.seed,
+                   "reductions": "fp64", "parallelism": "1 GPU"},
+        "value_frac_of_peak": round(value / HBM_PEAK_GBPS, 4),
+        "gate_layer_GBps": round(layers * 16 * dim / (avg_gate_ms / 1e3) / 1e9, 1),
         "gate_ms": round(avg_gate_ms, 3),
-        "gate_GBps": round(passes * gate_pass_bytes / (avg_gate_ms / 1e3) / 1e9, 1),
-        "gate_equiv_GBps": round(info["gates_after_merge"] * gate_pass_bytes / (avg_gate_ms / 1e3) / 1e9, 1),
         "layer_ms": round(avg_gate_ms / layers, 3),
         "expval_ms": round(float(np.mean(exp_ms)), 3),
+        "gate_passes": passes,
+        "expval_passes": int(ham.info()["passes"]),
+        "plan": {"tile_bits": info["tile_bits"], "phases": info["phases"],
+                 "gates_after_merge": info["gates_after_merge"], "max_cost": args.max_cost,
+                 "generated_kernels": args.jit >= 0, "build_s": round(build_s, 1)},
         "energy": energy,
         "norm2": norm,
         "roofline": {
@@ -190,18 +212,12 @@ def run_brickwork(args) -> dict:
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBPS, 4),
-            "traffic": None,
-            "kernel": "tqsv k_pass (fused tile pass)",
-            "bytes_per_launch": gate_pass_bytes,
+            "traffic": _traffic(f"gate_pass_{n}q"),
+            "kernel": "tqsv_gate_pass (fused tile pass, one state read + write)",
+            "bytes_per_launch": 16 * dim,
             "avg_launch_ms": round(per_launch_ms, 4),
         },
     }
-    if args.check:
-        ref = nat.Plan(n, ops, fuse=True, max_cost=100000)
-        nat.init_basis(psi, 0)
-        ref.run(psi)
-        e2, _ = ham.expval(psi)
-        out["check_energy_other_plan"] = e2
     if args.calib_copy:
         other = torch.empty_like(psi)
         other.copy_(psi)
@@ -212,28 +228,173 @@ def run_brickwork(args) -> dict:
     return out
 
 
-def ham_passes(ham) -> int:
-    """Read-only state sweeps of one expectation of a planned Hamiltonian."""
-    return int(ham.info()["passes"])
+# ---- config 3: adjoint energy+gradient vs the plain per-parameter path --------------
+def run_adjoint(args) -> dict:
+    import torch
+
+    from tyxonq_amd import _native as nat
+    from tyxonq_amd import bench_workloads as W
+    from tyxonq_amd.engine import lower
+    from tyxonq_amd.pauli import PauliSum
+
+    n = args.n or 22
+    layers = args.layers or 12
+    tt = torch.tensor(np.random.default_rng(args.seed).normal(0.0, 0.5, size=2 * n * layers),
+                      dtype=torch.float64, requires_grad=True)
+    circ, _ = W.ryrz_ring_ansatz(n, layers, params=[tt[i] for i in range(tt.numel())])
+    terms = W.heisenberg_ring_terms(n, 0.5)
+    L, _, _ = lower(circ, track_grad=True)
+    ops = L.segments[0][1]
+    t0 = time.perf_counter()
+    plan = nat.Plan(n, ops, max_cost=args.max_cost, jit=args.jit)
+    x, z, c = PauliSum(n, terms).to_masks()
+    ham = nat.Ham(n, x, z, c)
+    dev = nat.device()
+    psi = torch.empty(1 << n, dtype=torch.complex64, device=dev)
+    lam = torch.empty_like(psi)
+    nat.init_basis(psi, 0)
+    plan.adjoint(ham, psi, lam)  # builds the reverse plan
+    build_s = time.perf_counter() - t0
+    npar = plan.nparams
+    for _ in range(args.warmup):
+        nat.init_basis(psi, 0)
+        plan.adjoint(ham, psi, lam)
+    _sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        nat.init_basis(psi, 0)
+        e, g = plan.adjoint(ham, psi, lam)
+    _sync()
+    adj_s = (time.perf_counter() - t0) / args.steps
+
+    # plain path: parameter shift with the same kernels, 2 energies per parameter
+    base = ops.copy()
+    k = min(args.shift_sample, npar)
+    splan = nat.Plan(n, ops, max_cost=args.max_cost, jit=args.jit)
+    sel = np.nonzero(base["param"] >= 0)[0]
+
+    th = tt.detach().numpy()
+
+    def energy_with(row, delta):
+        """Energy with the parameter of op `row` shifted by delta (RY or RZ)."""
+        o = base.copy()
+        ang = th[int(o[row]["param"])] + delta
+        cs, sn = np.cos(ang / 2), np.sin(ang / 2)
+        if o[row]["gx"] != 0:  # RY: generator Y
+            m = [cs, 0.0, -sn, 0.0, sn, 0.0, cs, 0.0]
+        else:  # RZ: generator Z
+            m = [cs, -sn, 0.0, 0.0, 0.0, 0.0, cs, sn]
+        o[row]["m"][:8] = m
+        splan.update(o)
+        nat.init_basis(psi, 0)
+        splan.run(psi)
+        return ham.expval(psi)[0]
+
+    grad_ps = np.zeros(k)
+    energy_with(sel[0], 0.0)  # warm the update path
+    _sync()
+    t0 = time.perf_counter()
+    for j in range(k):
+        grad_ps[j] = 0.5 * (energy_with(sel[j], np.pi / 2) - energy_with(sel[j], -np.pi / 2))
+    _sync()
+    ps_per_param = (time.perf_counter() - t0) / k
+    plain_s = ps_per_param * npar + ps_per_param / 2  # 2P shifted energies + one energy
+    return {
+        "metric": METRIC,
+        "value": round(1.0 / adj_s, 3),
+        "unit": "energy+grad/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(adj_s * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64",
+        "data": "synthetic (seeded RY.RZ + CX-ring ansatz, periodic Heisenberg chain, hz=0.5)",
+        "config": {"workload": f"adjoint {n}q x {layers} layers, {npar} params, "
+                               f"{len(terms)}-term Heisenberg ring", "qubits": n,
+                   "layers": layers, "params": npar, "reductions": "fp64",
+                   "parallelism": "1 GPU"},
+        "energy": e,
+        "plain_path": {"kind": "parameter shift on the same kernels (2 energies per parameter)",
+                       "value": round(1.0 / plain_s, 5), "unit": "energy+grad/s",
+                       "timed_params": k, "s_per_param": round(ps_per_param, 4)},
+        "speedup_vs_plain": round(plain_s / adj_s, 1),
+        "max_abs_grad_diff_vs_shift_sample": float(np.max(np.abs(g[:k] - grad_ps))) if k else None,
+        "build_s": round(build_s, 1),
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+
+
+# ---- config 1: 12-qubit hardware-efficient ansatz, TFIM --------------------------
+def run_hea(args) -> dict:
+    import torch
+
+    import tyxonq_amd as tq
+    from tyxonq_amd import bench_workloads as W
+    from tyxonq_amd.engine import StatevectorEngine
+
+    n = args.n or 12
+    layers = args.layers or 4
+    th0 = np.random.default_rng(args.seed).uniform(0.0, 2.0 * np.pi, size=n * layers)
+    terms = W.tfim_terms(n, 1.0)
+    eng = StatevectorEngine(backend_name="pytorch")
+
+    def step():
+        tt = torch.tensor(th0, dtype=torch.float64, requires_grad=True)
+        c, _ = W.hea_ry_cx(n, layers, params=[tt[i] for i in range(tt.numel())])
+        return eng.energy_and_grad(c, terms)
+
+    for _ in range(args.warmup):
+        step()
+    _sync()
+    lat = []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        e, g = step()
+        lat.append(time.perf_counter() - t0)
+    lat = np.array(lat)
+    return {
+        "metric": METRIC,
+        "value": round(1.0 / float(lat.mean()), 2),
+        "unit": "energy+grad/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(float(lat.mean()) * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64",
+        "data": "synthetic (seeded RY + CX-ladder ansatz, open-chain TFIM g=1)",
+        "config": {"workload": f"hea {n}q x {layers} layers, TFIM energy+grad", "qubits": n,
+                   "layers": layers, "params": n * layers, "parallelism": "1 GPU"},
+        "energy": e,
+        "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
+        "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
+        "roofline": None,
+        "cpu_baseline": None,
+    }
 
 
 def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; --gpus must be 1 "
-                                   "(multi-GPU sharding is a design note, not a code path)"}))
+                                   "(how the state would shard over GPUs is a design note, "
+                                   "not a code path)"}))
         sys.exit(2)
-    if args.workload != "brickwork":
-        from tyxonq_amd import bench_modes
-
-        out = bench_modes.run(args)
+    if args.workload == "adjoint":
+        out = run_adjoint(args)
+    elif args.workload == "hea":
+        out = run_hea(args)
     else:
         out = run_brickwork(args)
-    if not args.no_cpu_baseline and args.workload == "brickwork":
-        out["cpu_baseline"] = cpu_baseline_brickwork(out["config"]["qubits"], args.seed)
-    elif "cpu_baseline" not in out:
-        out["cpu_baseline"] = None
-    print(json.dumps(out))
+        out["cpu_baseline"] = None if args.no_cpu_baseline else cpu_baseline_brickwork(
+            out["config"]["qubits"], out["config"]["layers"])
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

@@ -43,7 +43,10 @@ struct JitKernel {
 std::mutex g_mu;
 std::unordered_map<std::string, JitKernel> g_cache;  // device id + source -> kernel
 
-const char* kKernelName = "tqsv_jit_pass";
+// kernel names by mode, so that profiles tell the pass kinds apart
+const char* kernel_name(int mode) {
+  return mode == MODE_DUAL ? "tqsv_adjoint_pass" : mode == MODE_RED ? "tqsv_expval_pass" : "tqsv_gate_pass";
+}
 
 int pass_mode(uint32_t flags) {
   if (flags & PF_DUAL) return MODE_DUAL;
@@ -59,7 +62,7 @@ std::string gen_source(int tb, const uint32_t* pw, int n) {
   const int NT = 1 << tb, TSIZE = 1 << (tb + RB), NWAVE = std::max(NT / 64, 1);
   std::ostringstream o;
   o << "#include \"tqsv_device.h\"\nusing namespace tqsv;\n"
-    << "extern \"C\" __global__ void __launch_bounds__(" << NT << ") " << kKernelName
+    << "extern \"C\" __global__ void __launch_bounds__(" << NT << ") " << kernel_name(mode)
     << "(float2* __restrict__ psi, float2* __restrict__ lam, const uint32_t* __restrict__ prog_g,"
        " double* __restrict__ partials, uint32_t ntiles) {\n"
     << "  typedef " << (n > 32 ? "uint64_t" : "uint32_t") << " IDX;\n"
@@ -229,7 +232,8 @@ int jit_prepare(std::vector<PassPlan>& passes, const std::vector<uint32_t>& prog
     for (size_t j = 0; j < todo.size(); ++j) {
       JitKernel k;
       TQSV_HIP(hipModuleLoadData(&k.mod, codes[j].data()));
-      TQSV_HIP(hipModuleGetFunction(&k.fn, k.mod, kKernelName));
+      const uint32_t* pw = prog.data() + passes[todo[j]].prog_off;
+      TQSV_HIP(hipModuleGetFunction(&k.fn, k.mod, kernel_name(pass_mode(pw[PH_FLAGS]))));
       g_cache[keys[todo[j]]] = k;
     }
   }
