@@ -5,8 +5,7 @@
  * The reference has no native code and no FFI: its "kernels" are NumPy/PyTorch
  * einsums issued from Python.  Each entry point below replaces one reference
  * function (cited as file:line, relative to the reference's src/tyxonq/).  The
- * Python engine (tyxonq_amd/_native.py) binds these with ctypes; see
- * INTEGRATION.md for the binding a reference maintainer would add.
+ * Python engine (tyxonq_amd/_native.py) binds these with ctypes.
  *
  * Conventions
  *   - A state is 2^n complex64 amplitudes, interleaved (re, im) float pairs,
