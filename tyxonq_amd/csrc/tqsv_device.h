@@ -853,6 +853,68 @@ __device__ __forceinline__ void jit_write_partials(const double* acc, double* pa
   }
 }
 
+
+// ---- read-only expectation passes of generated kernels ----------------------
+// Every term of the pass has a per-thread FP64 accumulator that lives across
+// all tiles of the work-group: per tile the term's +-1 weighted sum over the
+// thread's 32 slots (FP32, at most 32 products) is added in FP64; the lanes and
+// waves are reduced once, at the end of the kernel.
+template <int XR, uint32_t KIND>
+__device__ __forceinline__ void jred_products(float2 (&p)[NSLOT], const float2 (&a)[NSLOT]) {
+  constexpr int LB = XR & (-XR);
+#pragma unroll
+  for (int s = 0; s < NSLOT; ++s) {
+    if constexpr (KIND == RED_PAIR) {
+      if (s & LB) continue;
+      const float2 u = a[s ^ XR], v = a[s];  // conj(u) * v
+      p[s] = make_float2(u.x * v.x + u.y * v.y, u.x * v.y - u.y * v.x);
+    } else {
+      p[s] = make_float2(a[s].x * a[s].x + a[s].y * a[s].y, 0.f);
+    }
+  }
+}
+
+// sum_s (-1)^{MASK bit s} part(p[s]) over the slots of the group (LB = lowest
+// flip bit: pairs are counted once), times (-1)^{|gb & zt|}
+template <uint32_t MASK, int PART, int LB, typename IDX>
+__device__ __forceinline__ double jred_term(const float2 (&p)[NSLOT], IDX gb, IDX zt) {
+  float v = 0.f;
+#pragma unroll
+  for (int s = 0; s < NSLOT; ++s) {
+    if (LB && (s & LB)) continue;
+    const float x = PART ? p[s].y : p[s].x;
+    v = ((MASK >> s) & 1u) ? v - x : v + x;
+  }
+  return (double)(parity((IDX)(gb & zt)) ? -v : v);
+}
+
+__device__ __forceinline__ double shfl_xor_d(double v, int o) {
+  const int lo = __shfl_xor(__double2loint(v), o, 64);
+  const int hi = __shfl_xor(__double2hiint(v), o, 64);
+  return __hiloint2double(hi, lo);
+}
+
+// lanes, then waves (through LDS), then one partial per term and work-group
+template <int K, int NWAVE>
+__device__ __forceinline__ void jred_finish(double (&acc)[K], double* lds, double* partials,
+                                            uint32_t tid, uint32_t nt) {
+  const uint32_t lane = tid & 63u, wave = tid >> 6;
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    double v = acc[k];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += shfl_xor_d(v, o);
+    if (lane == 0) lds[wave * K + k] = v;
+  }
+  __syncthreads();
+  for (uint32_t r = tid; r < (uint32_t)K; r += nt) {
+    double sum = 0.0;
+    for (int wv = 0; wv < NWAVE; ++wv) sum += lds[wv * K + r];
+    partials[(size_t)blockIdx.x * K + r] = sum;
+  }
+}
+
 }  // namespace tqsv
 
 #endif

@@ -24,6 +24,7 @@ using namespace tqsv;
 struct tqsv_ham {
   int n = 0;
   int nterms = 0;
+  int jit = 0;  // expectation passes run as generated kernels
   std::vector<uint64_t> x, z;
   std::vector<double> coeff;
   // expectation plan
@@ -78,9 +79,12 @@ int upload(T** dptr, const std::vector<T>& v) {
 }
 
 constexpr uint32_t NRED_MAX = 2048;
+// generated kernels keep one FP64 accumulator per term and thread in registers
+constexpr uint32_t NRED_MAX_GEN = 40;
 
 int plan_expval(tqsv_ham* h) {
   const int n = h->n;
+  const uint32_t cap = h->jit ? NRED_MAX_GEN : NRED_MAX;
   const int T = effective_tile_bits(n, 13, false);
   const bool tiles = T >= 11;
   const int nt = h->nterms + 1;  // + norm
@@ -103,18 +107,18 @@ int plan_expval(tqsv_ham* h) {
   if (tiles) {
     const int TB = T - RB;
     const int Lmin = std::min(5, T - RB);
-    // groups split into chunks of at most NRED_MAX/2 terms, most constrained first
+    // groups split into chunks of at most cap/2 terms, most constrained first
     struct Chunk {
       uint64_t x;
       std::vector<int> terms;
     };
     std::vector<Chunk> chunks;
     for (auto& kv : groups)
-      for (size_t i = 0; i < kv.second.size(); i += NRED_MAX / 2) {
+      for (size_t i = 0; i < kv.second.size(); i += std::max<uint32_t>(cap / 2, 1)) {
         Chunk ch;
         ch.x = kv.first;
         ch.terms.assign(kv.second.begin() + i,
-                        kv.second.begin() + std::min(kv.second.size(), i + NRED_MAX / 2));
+                        kv.second.begin() + std::min<size_t>(kv.second.size(), i + std::max<uint32_t>(cap / 2, 1)));
         chunks.push_back(ch);
       }
     std::stable_sort(chunks.begin(), chunks.end(), [&](const Chunk& a, const Chunk& b) {
@@ -132,7 +136,7 @@ int plan_expval(tqsv_ham* h) {
         if (done[i]) continue;
         const uint64_t need = order[i] & ~S;
         const uint32_t c = (uint32_t)chunks[i].terms.size();
-        if (popc(S | need) > T || nred + c > NRED_MAX) continue;
+        if (popc(S | need) > T || nred + c > cap) continue;
         S |= need;
         nred += c;
       }
@@ -164,7 +168,7 @@ int plan_expval(tqsv_ham* h) {
       for (size_t i = 0; i < order.size(); ++i) {
         if (done[i] || (order[i] & ~tileb)) continue;
         const uint32_t c = (uint32_t)chunks[i].terms.size();
-        if (nred + c > NRED_MAX) continue;
+        if (nred + c > cap) continue;
         mine.push_back(i);
         nred += c;
         done[i] = 1;
@@ -531,9 +535,10 @@ int tqsv_ham_create(int n, int nterms, const uint64_t* xmask, const uint64_t* zm
       h->z.push_back(zmask[t]);
       h->coeff.push_back(coeff[t]);
     }
+    h->jit = jit_policy(0, n);
     int rc = plan_expval(h.get());
     if (rc) return rc;
-    if (jit_policy(0, n)) {
+    if (h->jit) {
       rc = jit_prepare(h->passes, h->prog, n);
       if (rc) return rc;
     }

@@ -75,10 +75,17 @@ std::string gen_source(int tb, const uint32_t* pw, int n) {
     o << "  __shared__ __align__(16) float2 tB[" << TSIZE << "];\n";
   else
     o << "  float2* tB = nullptr;\n";
-  if (nred) o << "  __shared__ double acc[" << NWAVE * nred << "];\n";
+  // read-only passes keep per-thread accumulators in registers; dual passes
+  // accumulate per wave in LDS
+  const bool racc = mode == MODE_RED && nred > 0;
+  if (racc)
+    o << "  double racc[" << nred << "];\n#pragma unroll\n  for (int k = 0; k < " << nred
+      << "; ++k) racc[k] = 0.0;\n";
+  else if (nred)
+    o << "  __shared__ double acc[" << NWAVE * nred << "];\n";
   o << "  const cw_t prog = (cw_t)prog_g;\n"
     << "  const uint32_t tid = threadIdx.x;\n";
-  if (nred) o << "  jit_zero_acc<" << NWAVE << ", " << nred << ">(acc, tid, " << NT << ");\n";
+  if (nred && !racc) o << "  jit_zero_acc<" << NWAVE << ", " << nred << ">(acc, tid, " << NT << ");\n";
   o << "  for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {\n"
     << "    const IDX base = TM::base(tile);\n"
     << "    __syncthreads();\n"
@@ -127,6 +134,20 @@ std::string gen_source(int tb, const uint32_t* pw, int n) {
         }
         case OP_RED: {
           const uint32_t xr = w[1];
+          if (racc) {
+            const uint32_t kind = xr ? RED_PAIR : RED_DIAG;
+            const uint32_t lb = xr & (~xr + 1u);
+            o << "      {\n        float2 p[NSLOT];\n        jred_products<" << xr << ", " << kind
+              << ">(p, a);\n";
+            const uint32_t* e = w + RED_HDR;
+            for (uint32_t t = 0; t < w[3]; ++t, e += RED_ENT) {
+              const uint64_t zt = (uint64_t)e[1] | ((uint64_t)e[2] << 32);
+              o << "        racc[" << e[0] << "] += jred_term<" << e[3] << "u, " << e[4] << ", "
+                << lb << ", IDX>(p, G.gb, (IDX)" << zt << "ull);\n";
+            }
+            o << "      }\n";
+            break;
+          }
           if (mode == MODE_DUAL)
             o << "      red_cross<" << xr << ", IDX>(a, b, " << P << ", G, acc, " << nred << ");\n";
           else
@@ -144,7 +165,11 @@ std::string gen_source(int tb, const uint32_t* pw, int n) {
   }
   if (write) o << "    jit_store_tile<" << tb << ", MODE, IDX, TM>(psi, lam, tA, tB, base, tid);\n";
   o << "  }\n";
-  if (nred) o << "  jit_write_partials<" << NWAVE << ", " << nred << ">(acc, partials, tid, " << NT << ");\n";
+  if (racc)
+    o << "  jred_finish<" << nred << ", " << NWAVE << ">(racc, reinterpret_cast<double*>(tA), partials, tid, "
+      << NT << ");\n";
+  else if (nred)
+    o << "  jit_write_partials<" << NWAVE << ", " << nred << ">(acc, partials, tid, " << NT << ");\n";
   o << "}\n";
   return o.str();
 }

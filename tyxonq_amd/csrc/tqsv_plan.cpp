@@ -846,8 +846,71 @@ int schedule(std::vector<Gate>& gates, const SchedOpts& so, std::vector<PassPlan
       rem.erase(rem.begin());
       continue;
     }
-    // 1) choose the tile's bit set
+    // 1) choose the tile's bit set.  Search: grow the set from the low bits by
+    // the candidate (one bit, or the non-tile bits of an upcoming gate) that
+    // admits the most work per added bit under the rule of step 2, ties broken
+    // by the work admissible without the budget (light-cone reach); then fill
+    // the remaining bits in gate order.
     uint64_t S = lowmask(Lmin);
+    if (so.tile_search) {
+      auto work = [&](uint64_t bits, bool use_budget) -> long {
+        uint64_t bN = 0, bD = 0;
+        int bud = budget0;
+        long w = 0;
+        bool first = true;
+        for (int gi : rem) {
+          const Gate& g = gates[gi];
+          if (bN == full) break;
+          if (conflicts(g, bN, bD) || !fusable(g) || (g.nb & ~bits) ||
+              (use_budget && !first && g.cost > bud)) {
+            block(g, bN, bD);
+            continue;
+          }
+          w += g.cost;
+          bud -= g.cost;
+          first = false;
+        }
+        return w;
+      };
+      long wcur = work(S, true);
+      for (;;) {
+        const int room = T - popc(S);
+        if (room <= 0) break;
+        std::vector<uint64_t> cands;
+        for (int b = 0; b < n; ++b)
+          if (!((S >> b) & 1u)) cands.push_back(bitm(b));
+        uint64_t bN = 0, bD = 0;
+        size_t seen = 0;
+        for (int gi : rem) {
+          const Gate& g = gates[gi];
+          if (bN == full || seen >= 64) break;
+          const uint64_t add = g.nb & ~S;
+          if (!conflicts(g, bN, bD) && fusable(g) && popc(add) >= 2 && popc(add) <= room &&
+              std::find(cands.begin(), cands.end(), add) == cands.end()) {
+            cands.push_back(add);
+            ++seen;
+          }
+          block(g, bN, bD);
+        }
+        uint64_t best = 0;
+        double bscore = -1.0;
+        long bpot = -1, bw = 0;
+        for (uint64_t c : cands) {
+          const long w = work(S | c, true);
+          const long pot = work(S | c, false);
+          const double score = (double)(w - wcur) / popc(c);
+          if (score > bscore || (score == bscore && pot > bpot)) {
+            best = c;
+            bscore = score;
+            bpot = pot;
+            bw = w;
+          }
+        }
+        if (!best || (bscore <= 0.0 && bpot <= wcur)) break;
+        S |= best;
+        wcur = bw;
+      }
+    }
     uint64_t blkN = 0, blkD = 0;
     int budget = budget0;
     bool any = false;  // the first admissible gate is taken whatever its cost

@@ -82,6 +82,7 @@ struct SchedOpts {
   int max_cost = 96;
   bool fuse = true;
   bool dual = false;
+  bool tile_search = false;  // choose tile bits by admitted work (else in gate order)
 };
 
 // Classify one gate (usage bits, diagonal forms, cost).
