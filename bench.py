@@ -45,7 +45,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="brickwork", choices=["brickwork", "adjoint", "hea"])
+    ap.add_argument("--workload", default="brickwork",
+                    choices=["brickwork", "adjoint", "hea", "ucc"])
     ap.add_argument("--n", type=int, default=None)
     ap.add_argument("--layers", type=int, default=None)
     ap.add_argument("--terms", type=int, default=500)
@@ -379,6 +380,99 @@ def run_hea(args) -> dict:
     }
 
 
+# ---- config 4: UCCSD, 20 spin orbitals ----------------------------------------
+def run_ucc(args) -> dict:
+    """One step = new parameters -> energy and dE/dtheta for the 450 shared
+    parameters: the parameter update re-emits the plan's programs (selection
+    kept), then one forward and one adjoint sweep.  The same step through the
+    Circuit / torch.autograd surface is timed as `autograd_path`."""
+    import torch
+
+    from tyxonq_amd import _native as nat
+    from tyxonq_amd import chem
+    from tyxonq_amd.engine import StatevectorEngine
+    from tyxonq_amd.pauli import PauliSum
+
+    ns = (args.n or 20) // 2
+    no = ns // 2
+    nv = ns - no
+    n = 2 * ns
+    t0 = time.perf_counter()
+    h, g = chem.synthetic_integrals(ns, args.seed)
+    terms = chem.molecular_hamiltonian(h, g)
+    ex, ids = chem.uccsd_excitations(no, nv)
+    ans = chem.UCCAnsatz(n, (no, no), ex, ids)
+    npar = ans.nparams
+    rng = np.random.default_rng(args.seed)
+    thetas = [rng.normal(0.0, 0.05, size=npar) for _ in range(args.warmup + args.steps)]
+    x, z, c = PauliSum(n, terms).to_masks()
+    ham = nat.Ham(n, x, z, c)
+    plan = nat.Plan(n, ans.ops(thetas[0]), tile_bits=args.tile_bits or 13, max_cost=100000,
+                    jit=args.jit)
+    psi = torch.empty(1 << n, dtype=torch.complex64, device=nat.device())
+    lam = torch.empty_like(psi)
+    nat.init_basis(psi, 0)
+    plan.adjoint(ham, psi, lam)
+    build_s = time.perf_counter() - t0
+
+    def step(th):
+        plan.update(ans.ops(th))
+        nat.init_basis(psi, 0)
+        return plan.adjoint(ham, psi, lam)
+
+    for k in range(args.warmup):
+        step(thetas[k])
+    _sync()
+    lat = []
+    for k in range(args.steps):
+        t0 = time.perf_counter()
+        e, gr = step(thetas[args.warmup + k])
+        lat.append(time.perf_counter() - t0)
+    lat = np.array(lat)
+    info = plan.info()
+
+    # the same energy+gradient through Circuit + torch.autograd (the drop-in surface)
+    eng = StatevectorEngine(backend_name="pytorch")
+    tt = torch.tensor(thetas[-1], dtype=torch.float64, requires_grad=True)
+    circ = chem.ucc_circuit([tt[i] for i in range(npar)], n, (no, no), ex, ids)
+    ea = eng.energy(circ, terms)
+    ea.backward()
+    t0 = time.perf_counter()
+    tt = torch.tensor(thetas[-1], dtype=torch.float64, requires_grad=True)
+    circ = chem.ucc_circuit([tt[i] for i in range(npar)], n, (no, no), ex, ids)
+    ea = eng.energy(circ, terms)
+    ea.backward()
+    auto_s = time.perf_counter() - t0
+    return {
+        "metric": METRIC,
+        "value": round(1.0 / float(lat.mean()), 3),
+        "unit": "energy+grad/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(float(lat.mean()) * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64",
+        "data": "synthetic (seeded integrals, HF reference, UCCSD theta ~ N(0, 0.05^2))",
+        "config": {"workload": f"ucc {n} spin orbitals, {2 * no} electrons, {len(ex)} excitations "
+                               f"({npar} params), {ans.nrot} Pauli rotations, {len(terms)}-term JW "
+                               f"Hamiltonian", "qubits": n, "params": npar, "rotations": ans.nrot,
+                   "pauli_terms": len(terms), "reductions": "fp64", "parallelism": "1 GPU"},
+        "energy": e,
+        "grad_norm": float(np.linalg.norm(gr)),
+        "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
+        "gate_passes": info["passes"],
+        "autograd_path": {"ms": round(auto_s * 1e3, 1),
+                          "abs_dE_vs_direct": abs(float(ea.detach()) - e) if args.steps else None,
+                          "max_abs_dgrad_vs_direct": float(np.max(np.abs(tt.grad.numpy() - gr)))},
+        "build_s": round(build_s, 1),
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -390,6 +484,8 @@ def main():
         out = run_adjoint(args)
     elif args.workload == "hea":
         out = run_hea(args)
+    elif args.workload == "ucc":
+        out = run_ucc(args)
     else:
         out = run_brickwork(args)
         out["cpu_baseline"] = None if args.no_cpu_baseline else cpu_baseline_brickwork(

@@ -242,7 +242,7 @@ int tqsv_plan_create(int n, const tqsv_op* ops, int nops, const tqsv_plan_opts* 
       p->so.max_cost = opts->max_cost > 0 ? opts->max_cost : 96;
       p->so.fuse = opts->fuse != 0;
     }
-    p->jit = jit_policy(opts ? opts->jit : 0, n);
+    p->jit_req = opts ? opts->jit : 0;
     const int rc = plan_build_forward(p.get(), ops, nops, nullptr);
     if (rc) return rc;
     *out = p.release();

@@ -85,18 +85,29 @@ __device__ __forceinline__ uint32_t ls_of(const PhaseGeo<IDX>& G, int s) {
   return v;
 }
 
-// ---- wave reduction (FP32 butterfly over the 64 lanes) ----------------------
+// ---- wave reduction -------------------------------------------------------
+// Butterfly over the 64 lanes: quad_perm [1,0,3,2], quad_perm [2,3,0,1],
+// row_half_mirror (lanes i and 7-i of each 8), row_mirror (i and 15-i of each
+// 16), then exchanges with lanes i^16 and i^32.
+// FP64 butterfly: the same lane pattern, each double moved as two 32-bit halves
 template <int CTRL>
-__device__ __forceinline__ float dpp_f(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
+__device__ __forceinline__ double dpp_d(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
 }
-__device__ __forceinline__ float wave_sum_f(float v) {
-  v += dpp_f<0xB1>(v);   // quad_perm [1,0,3,2]
-  v += dpp_f<0x4E>(v);   // quad_perm [2,3,0,1]
-  v += dpp_f<0x141>(v);  // row_half_mirror: lanes i and 7-i of each 8
-  v += dpp_f<0x140>(v);  // row_mirror: lanes i and 15-i of each 16
-  v += __shfl_xor(v, 16, 64);
-  v += __shfl_xor(v, 32, 64);
+__device__ __forceinline__ double shfl_xor_d(double v, int o) {
+  const int lo = __shfl_xor(__double2loint(v), o, 64);
+  const int hi = __shfl_xor(__double2hiint(v), o, 64);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double wave_sum_d(double v) {
+  v += dpp_d<0xB1>(v);
+  v += dpp_d<0x4E>(v);
+  v += dpp_d<0x141>(v);
+  v += dpp_d<0x140>(v);
+  v += shfl_xor_d(v, 16);
+  v += shfl_xor_d(v, 32);
   return v;
 }
 
@@ -476,10 +487,11 @@ __device__ __forceinline__ float sgn1(uint32_t mask, int s) {
   return __uint_as_float(0x3f800000u | (((mask >> s) & 1u) << 31));
 }
 
-// Per term: FP32 sum over the thread's slots and the wave's 64 lanes (2048
-// products), then FP64 accumulation in LDS (per wave) and beyond.
+// Per term: FP32 sum over the thread's 32 slots, then FP64 over the wave's
+// lanes, the LDS accumulator of the wave and beyond.
 // Cross group of a dual pass: exactly two entries (real and imaginary part)
-// with the same z; both sums are formed in one sweep over the slots.
+// with the same z; both sums are formed in one sweep over the slots (FP32,
+// 32 products per thread) and reduced over the wave in FP64.
 template <int XR, typename IDX>
 __device__ __forceinline__ void red_cross(const float2 (&a)[NSLOT], const float2 (&b)[NSLOT],
                                           cw_t w, const PhaseGeo<IDX>& G, double* acc,
@@ -499,12 +511,12 @@ __device__ __forceinline__ void red_cross(const float2 (&a)[NSLOT], const float2
     vr = -vr;
     vi = -vi;
   }
-  vr = wave_sum_f(vr);
-  vi = wave_sum_f(vi);
+  const double dr = wave_sum_d((double)vr);
+  const double di = wave_sum_d((double)vi);
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
   if (lane == 0) {
-    acc[wave * nred + e[0]] += (double)vr;
-    acc[wave * nred + e[RED_ENT]] += (double)vi;
+    acc[wave * nred + e[0]] += dr;
+    acc[wave * nred + e[RED_ENT]] += di;
   }
 }
 
@@ -551,8 +563,8 @@ __device__ __forceinline__ void red_x(const float2 (&a)[NSLOT], const float2 (&b
       }
     }
     v = parity((IDX)(G.gb & zt)) ? -v : v;
-    v = wave_sum_f(v);
-    if (lane == 0) acc[wave * nred + slot] += (double)v;
+    const double d = wave_sum_d((double)v);
+    if (lane == 0) acc[wave * nred + slot] += d;
   }
 }
 
@@ -886,12 +898,6 @@ __device__ __forceinline__ double jred_term(const float2 (&p)[NSLOT], IDX gb, ID
     v = ((MASK >> s) & 1u) ? v - x : v + x;
   }
   return (double)(parity((IDX)(gb & zt)) ? -v : v);
-}
-
-__device__ __forceinline__ double shfl_xor_d(double v, int o) {
-  const int lo = __shfl_xor(__double2loint(v), o, 64);
-  const int hi = __shfl_xor(__double2hiint(v), o, 64);
-  return __hiloint2double(hi, lo);
 }
 
 // lanes, then waves (through LDS), then one partial per term and work-group

@@ -286,16 +286,21 @@ int plan_expval(tqsv_ham* h) {
   h->partials_cap = std::max(h->partials_cap, (size_t)reduce_grid(n) * 2);
   h->nout = nout;
   h->hout.assign(nout, 0.0);
+  return 0;
+}
+
+// device copies of the expectation plan (after planning is final)
+int upload_expval(tqsv_ham* h) {
   if (upload(&h->dprog, h->prog)) return TQSV_EDEVICE;
   std::vector<uint64_t> sx, sz;
   for (int t : h->small_terms) {
-    sx.push_back(tx(t));
-    sz.push_back(tz(t));
+    sx.push_back(t < h->nterms ? h->x[t] : 0ull);
+    sz.push_back(t < h->nterms ? h->z[t] : 0ull);
   }
   if (upload(&h->d_sx, sx)) return TQSV_EDEVICE;
   if (upload(&h->d_sz, sz)) return TQSV_EDEVICE;
   TQSV_HIP(hipMalloc(reinterpret_cast<void**>(&h->dpartials), h->partials_cap * sizeof(double)));
-  TQSV_HIP(hipMalloc(reinterpret_cast<void**>(&h->dout), std::max<uint32_t>(nout, 1) * sizeof(double)));
+  TQSV_HIP(hipMalloc(reinterpret_cast<void**>(&h->dout), std::max<uint32_t>(h->nout, 1) * sizeof(double)));
   return 0;
 }
 
@@ -404,21 +409,36 @@ int plan_upload(uint32_t** dptr, size_t* cap, const std::vector<uint32_t>& prog,
   return 0;
 }
 
+static size_t count_tile(const std::vector<PassPlan>& passes) {
+  size_t k = 0;
+  for (const PassPlan& pp : passes) k += pp.tile ? 1 : 0;
+  return k;
+}
+
 int plan_build_forward(tqsv_plan* p, const tqsv_op* ops, int nops, hipStream_t st) {
   std::vector<Gate> g;
   int rc = gates_from_ops(p->n, ops, nops, g);
   if (rc) return rc;
   merge_and_classify(g);
+  const bool same = same_structure(g, p->gates) && !p->fsel.empty();
   if (g.size() == p->gates.size())
     for (size_t i = 0; i < g.size(); ++i) g[i].rot_form = p->gates[i].rot_form;
+  std::vector<PassPlan> sel;
+  if (same) {
+    sel = p->fsel;
+  } else {
+    rc = select_passes(g, p->so, sel);
+    if (rc) return rc;
+  }
   std::vector<PassPlan> passes;
   std::vector<uint32_t> prog;
   uint32_t red_total = 0;
-  rc = schedule(g, p->so, passes, prog, red_total);
+  rc = emit_passes(g, p->so, sel, passes, prog, red_total);
   if (rc) return rc;
+  p->fsel.swap(sel);
   p->max_param = -1;
   for (const Gate& x : g) p->max_param = std::max(p->max_param, x.param);
-  if (p->jit) {
+  if (jit_policy(p->jit_req, p->n, count_tile(passes), 64)) {
     rc = jit_prepare(passes, prog, p->n);
     if (rc) return rc;
   }
@@ -457,16 +477,26 @@ int plan_build_adjoint(tqsv_plan* p, hipStream_t st) {
     }
     r.push_back(dagger(g));
   }
+  const bool same = same_structure(r, p->rgates) && !p->rsel.empty();
   if (r.size() == p->rgates.size())
     for (size_t i = 0; i < r.size(); ++i) r[i].rot_form = p->rgates[i].rot_form;
   SchedOpts so = p->so;
   so.dual = true;
+  std::vector<PassPlan> sel;
+  int rc = 0;
+  if (same) {
+    sel = p->rsel;
+  } else {
+    rc = select_passes(r, so, sel);
+    if (rc) return rc;
+  }
   std::vector<PassPlan> passes;
   std::vector<uint32_t> prog;
   uint32_t red_total = 0;
-  int rc = schedule(r, so, passes, prog, red_total);
+  rc = emit_passes(r, so, sel, passes, prog, red_total);
   if (rc) return rc;
-  if (p->jit) {
+  p->rsel.swap(sel);
+  if (jit_policy(p->jit_req, p->n, count_tile(passes), 64)) {
     rc = jit_prepare(passes, prog, p->n);
     if (rc) return rc;
   }
@@ -535,8 +565,23 @@ int tqsv_ham_create(int n, int nterms, const uint64_t* xmask, const uint64_t* zm
       h->z.push_back(zmask[t]);
       h->coeff.push_back(coeff[t]);
     }
-    h->jit = jit_policy(0, n);
+    // generated kernels hold at most NRED_MAX_GEN terms per pass; when that
+    // gives too many passes (compile time), plan for the precompiled kernel
+    h->jit = jit_policy(0, n, 0, 32);
     int rc = plan_expval(h.get());
+    if (rc) return rc;
+    if (h->jit && !jit_policy(0, n, h->passes.size(), 32)) {
+      h->jit = 0;
+      h->passes.clear();
+      h->prog.clear();
+      h->small_terms.clear();
+      h->big_terms.clear();
+      h->nout = 0;
+      h->partials_cap = 0;
+      rc = plan_expval(h.get());
+      if (rc) return rc;
+    }
+    rc = upload_expval(h.get());
     if (rc) return rc;
     if (h->jit) {
       rc = jit_prepare(h->passes, h->prog, n);

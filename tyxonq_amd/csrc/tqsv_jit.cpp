@@ -205,14 +205,16 @@ bool compile_source(const std::string& src, std::vector<char>& code, std::string
 
 }  // namespace
 
-int jit_policy(int requested, int n) {
+int jit_policy(int requested, int n, size_t tile_passes, size_t max_auto_passes) {
   if (const char* e = std::getenv("TQSV_JIT")) {
     if (e[0] == '0') return 0;
     if (e[0] == '1') return 1;
   }
   if (requested > 0) return 1;
   if (requested < 0) return 0;
-  return n >= 20 ? 1 : 0;
+  // automatic: states of 2^22+ amplitudes, and plans whose compile time
+  // (about a second per distinct pass) stays moderate
+  return n >= 22 && tile_passes <= max_auto_passes ? 1 : 0;
 }
 
 std::string jit_source(int tb, const uint32_t* pass_words, int n) { return gen_source(tb, pass_words, n); }

@@ -249,7 +249,7 @@ __global__ void ham_apply_kernel(const float2* psi, float2* out, uint64_t dim, u
                                  const float2* tc) {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < dim; i += stride) {
-    float ar = 0.f, ai = 0.f;
+    double ar = 0.0, ai = 0.0;  // FP64 across the groups, one rounding at the end
     for (uint32_t g = 0; g < ngroups; ++g) {
       const uint64_t j = i ^ gx[g];
       const float2 v = psi[j];
@@ -259,10 +259,10 @@ __global__ void ham_apply_kernel(const float2* psi, float2* out, uint64_t dim, u
         fr += sg * tc[t].x;
         fi += sg * tc[t].y;
       }
-      ar += fr * v.x - fi * v.y;
-      ai += fr * v.y + fi * v.x;
+      ar += (double)(fr * v.x) - (double)(fi * v.y);
+      ai += (double)(fr * v.y) + (double)(fi * v.x);
     }
-    out[i] = make_float2(ar, ai);
+    out[i] = make_float2((float)ar, (float)ai);
   }
 }
 

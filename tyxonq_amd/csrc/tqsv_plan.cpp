@@ -813,8 +813,7 @@ int effective_tile_bits(int n, int requested, bool dual) {
   return T;
 }
 
-int schedule(std::vector<Gate>& gates, const SchedOpts& so, std::vector<PassPlan>& passes,
-             std::vector<uint32_t>& prog, uint32_t& red_total) {
+int select_passes(std::vector<Gate>& gates, const SchedOpts& so, std::vector<PassPlan>& plan) {
   const int n = so.n;
   const int T = effective_tile_bits(n, so.tile_bits, so.dual);
   const bool tiles_ok = so.fuse && T >= 11 && !(so.dual && n > 32);
@@ -826,12 +825,16 @@ int schedule(std::vector<Gate>& gates, const SchedOpts& so, std::vector<PassPlan
   if (Lmin < 1) Lmin = 1;
   const uint64_t full = lowmask(n);
   const int budget0 = so.max_cost > 0 ? so.max_cost : 96;
+  // reduction slots per pass: the LDS beyond the tile(s) holds one double per
+  // slot and wave
+  const size_t lds_tiles = pass_lds_bytes(TBv, mode, 0);
+  const size_t nwave = ((size_t)1 << TBv) >= 64 ? ((size_t)1 << TBv) / 64 : 1;
+  const uint32_t red_cap =
+      lds_tiles < 160 * 1024 ? (uint32_t)((160 * 1024 - lds_tiles) / (nwave * sizeof(double))) : 0;
 
   std::vector<int> rem(gates.size());
   for (size_t i = 0; i < gates.size(); ++i) rem[i] = (int)i;
   std::vector<char> done(gates.size(), 0);
-  std::vector<PassPlan> plan;
-  std::vector<TileGeo> geos;
 
   // ---- pass and phase selection
   while (!rem.empty()) {
@@ -842,7 +845,6 @@ int schedule(std::vector<Gate>& gates, const SchedOpts& so, std::vector<PassPlan
       pp.tile = false;
       pp.single = rem[0];
       plan.push_back(pp);
-      geos.emplace_back();
       rem.erase(rem.begin());
       continue;
     }
@@ -950,19 +952,23 @@ int schedule(std::vector<Gate>& gates, const SchedOpts& so, std::vector<PassPlan
     if ((int)geo.H.size() > MAXH || L < 1 || L + (int)geo.H.size() != T)
       return fail(TQSV_EINVAL, "planner built an invalid tile (internal error)");
     const uint64_t tilebits = lowmask(L) | (S & ~lowmask(L));
-    // 2) take every admissible gate with the tile fixed
+    // 2) take every admissible gate with the tile fixed; reductions also
+    // count against the LDS left for their per-wave accumulators
     std::vector<int> chosen;
     blkN = blkD = 0;
     budget = budget0;
+    uint32_t nred_pass = 0;
     for (int gi : rem) {
       const Gate& g = gates[gi];
       if (conflicts(g, blkN, blkD) || !fusable(g) || (g.nb & ~tilebits) ||
-          (!chosen.empty() && g.cost > budget)) {
+          (!chosen.empty() && g.cost > budget) ||
+          (g.kind == GK_RED && nred_pass + 2 > red_cap)) {
         block(g, blkN, blkD);
         continue;
       }
       chosen.push_back(gi);
       budget -= g.cost;
+      if (g.kind == GK_RED) nred_pass += 2;
     }
     if (chosen.empty()) return fail(TQSV_EINVAL, "planner made no progress (internal error)");
     // 3) phases
@@ -1016,9 +1022,17 @@ int schedule(std::vector<Gate>& gates, const SchedOpts& so, std::vector<PassPlan
     pp.m = (uint32_t)geo.H.size();
     for (size_t k = 0; k < geo.H.size(); ++k) pp.H[k] = (uint32_t)geo.H[k];
     plan.push_back(pp);
-    geos.push_back(geo);
   }
+  return 0;
+}
 
+
+int emit_passes(std::vector<Gate>& gates, const SchedOpts& so, std::vector<PassPlan> plan,
+                std::vector<PassPlan>& passes, std::vector<uint32_t>& prog, uint32_t& red_total) {
+  const int n = so.n;
+  const int T = effective_tile_bits(n, so.tile_bits, so.dual);
+  const int TBv = T - RB;
+  const int mode = so.dual ? MODE_DUAL : MODE_GATE;
   // ---- emission.  The pending diagonal travels from pass to pass; it is
   // flushed in the last phase of the last tile pass before a single-op pass
   // (which does not know about it) and at the end of the plan.
@@ -1038,7 +1052,10 @@ int schedule(std::vector<Gate>& gates, const SchedOpts& so, std::vector<PassPlan
       continue;
     }
     const bool flush = i + 1 == plan.size() || !plan[i + 1].tile;
-    const TileGeo& geo = geos[i];
+    TileGeo geo;
+    geo.T = T;
+    geo.L = (int)pp.L;
+    for (uint32_t k = 0; k < pp.m; ++k) geo.H.push_back((int)pp.H[k]);
     pp.prog_off = prog.size();
     pp.red_base = red_total;
     Emit e(prog);
@@ -1077,6 +1094,27 @@ int schedule(std::vector<Gate>& gates, const SchedOpts& so, std::vector<PassPlan
   }
   if (!P.empty()) return fail(TQSV_EINVAL, "planner left diagonal factors unapplied (internal error)");
   return 0;
+}
+
+int schedule(std::vector<Gate>& gates, const SchedOpts& so, std::vector<PassPlan>& passes,
+             std::vector<uint32_t>& prog, uint32_t& red_total) {
+  std::vector<PassPlan> plan;
+  const int rc = select_passes(gates, so, plan);
+  if (rc) return rc;
+  return emit_passes(gates, so, std::move(plan), passes, prog, red_total);
+}
+
+bool same_structure(const std::vector<Gate>& a, const std::vector<Gate>& b) {
+  if (a.size() != b.size()) return false;
+  for (size_t i = 0; i < a.size(); ++i) {
+    const Gate& x = a[i];
+    const Gate& y = b[i];
+    if (x.kind != y.kind || x.t0 != y.t0 || x.t1 != y.t1 || x.cmask != y.cmask ||
+        x.cval != y.cval || x.x != y.x || x.z != y.z || x.nb != y.nb || x.db != y.db ||
+        x.cost != y.cost || (x.param >= 0) != (y.param >= 0))
+      return false;
+  }
+  return true;
 }
 
 static hipError_t apply_single(const Gate& g, int n, float2* psi, hipStream_t st) {

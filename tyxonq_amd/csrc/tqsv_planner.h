@@ -94,6 +94,15 @@ void classify_gate(Gate& g);
 int schedule(std::vector<Gate>& gates, const SchedOpts& so, std::vector<PassPlan>& passes,
              std::vector<uint32_t>& prog, uint32_t& red_total);
 
+// The two halves of schedule(): pass/phase selection (which gates go to which
+// pass and phase, tile and register bits) and emission of the programs.  A plan
+// whose gates keep their structure (same_structure) when only parameters
+// change reuses its selection and re-emits.
+int select_passes(std::vector<Gate>& gates, const SchedOpts& so, std::vector<PassPlan>& plan);
+int emit_passes(std::vector<Gate>& gates, const SchedOpts& so, std::vector<PassPlan> plan,
+                std::vector<PassPlan>& passes, std::vector<uint32_t>& prog, uint32_t& red_total);
+bool same_structure(const std::vector<Gate>& a, const std::vector<Gate>& b);
+
 // Run planned passes (device program at dprog; partial-sum scratch for reductions).
 int run_passes(const std::vector<PassPlan>& passes, const std::vector<Gate>& gates, int n,
                const uint32_t* dprog, float2* psi, float2* lam, double* dpartials,
@@ -102,9 +111,10 @@ int run_passes(const std::vector<PassPlan>& passes, const std::vector<Gate>& gat
 int effective_tile_bits(int n, int requested, bool dual);
 
 // ---- generated pass kernels (tqsv_jit.cpp) ----
-// 1 if plans of n qubits use generated kernels (requested: 1 on, -1 off, 0
-// automatic; the environment variable TQSV_JIT=0/1 overrides).
-int jit_policy(int requested, int n);
+// 1 if a plan of n qubits with the given number of tile passes uses
+// generated kernels (requested: 1 on, -1 off, 0 automatic: n >= 22 and at most
+// max_auto_passes passes; the environment variable TQSV_JIT=0/1 overrides).
+int jit_policy(int requested, int n, size_t tile_passes, size_t max_auto_passes);
 // Compile (or take from the process-wide cache) the kernel of every tile pass
 // and set PassPlan::jit_fn.
 int jit_prepare(std::vector<PassPlan>& passes, const std::vector<uint32_t>& prog, int n);
@@ -117,9 +127,10 @@ std::string jit_source(int tb, const uint32_t* pass_words, int n);
 struct tqsv_plan {
   int n = 0;
   int nops = 0;
-  int jit = 0;  // passes run as generated kernels
+  int jit_req = 0;  // requested: 1 generated kernels, -1 precompiled, 0 automatic
   tqsv::SchedOpts so;
   std::vector<tqsv::Gate> gates;          // forward gates after merge
+  std::vector<tqsv::PassPlan> fsel;       // pass/phase selection of the forward gates
   std::vector<tqsv::PassPlan> passes;
   std::vector<uint32_t> prog;
   uint32_t* dprog = nullptr;
@@ -127,6 +138,7 @@ struct tqsv_plan {
   // adjoint (built lazily)
   bool adj_built = false;
   std::vector<tqsv::Gate> rgates;         // reverse sweep items
+  std::vector<tqsv::PassPlan> rsel;
   std::vector<tqsv::PassPlan> rpasses;
   std::vector<uint32_t> rprog;
   uint32_t* drprog = nullptr;
