@@ -46,7 +46,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="brickwork",
-                    choices=["brickwork", "adjoint", "hea", "ucc"])
+                    choices=["brickwork", "adjoint", "hea", "ucc", "serve"])
     ap.add_argument("--n", type=int, default=None)
     ap.add_argument("--layers", type=int, default=None)
     ap.add_argument("--terms", type=int, default=500)
@@ -58,6 +58,10 @@ def parse():
     ap.add_argument("--calib-copy", action="store_true",
                     help="after the timed steps, copy the state once with a device copy "
                          "kernel of known size (calibration of HBM byte counters)")
+    ap.add_argument("--rates", default="50,100,200,500,1000,2000",
+                    help="serve workload: offered request rates (req/s)")
+    ap.add_argument("--requests", type=int, default=120, help="serve workload: requests per rate")
+    ap.add_argument("--slots", type=int, default=8, help="serve workload: requests in flight")
     ap.add_argument("--shift-sample", type=int, default=8,
                     help="adjoint workload: parameters timed on the parameter-shift path")
     return ap.parse_args()
@@ -473,6 +477,55 @@ def run_ucc(args) -> dict:
     }
 
 
+# ---- config 5: serving with continuous batching ---------------------------------
+def run_serve(args) -> dict:
+    """Per offered rate: the same Poisson stream served by the continuous-
+    batching server and by the plain one-request-at-a-time loop."""
+    import torch
+
+    from tyxonq_amd import serving as S
+
+    rates = [float(r) for r in args.rates.split(",")]
+    warm = S.make_requests(2 * args.slots, 1e6, seed=args.seed + 99)
+    S.Server(slots=args.slots).run(warm, realtime=False)
+    S.plain_loop(warm[:4], realtime=False)
+    table = []
+    for k, rate in enumerate(rates):
+        reqs = S.make_requests(args.requests, rate, seed=args.seed + k)
+        srv = S.summarize(S.Server(slots=args.slots).run(reqs))
+        plain = S.summarize(S.plain_loop(reqs))
+        table.append({"rate_rps": rate, "server": srv, "plain": plain})
+        print(json.dumps({"rate": rate, "server": srv, "plain": plain}), file=sys.stderr, flush=True)
+    top = table[-1]
+    torch.cuda.synchronize()
+    return {
+        "metric": METRIC,
+        "value": top["server"]["throughput_rps"],
+        "unit": "energy+grad requests/s",
+        "n_gpus": 1,
+        "steps": args.requests,
+        "warmup": len(warm),
+        "ms_per_step": round(1e3 / top["server"]["throughput_rps"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64",
+        "data": "synthetic (seeded Poisson arrivals; n ~ U{10..26}, depth ~ U{2..16}, "
+                "RY/RZ + CX-ladder ansatz, TFIM or Heisenberg with g ~ U[0.5, 1.5])",
+        "config": {"workload": f"serving at {top['rate_rps']:.0f} req/s offered "
+                               f"({args.requests} requests per rate), adjoint energy+grad",
+                   "slots": args.slots, "admission": "2^n x 24 B footprint",
+                   "parallelism": "1 GPU, one HIP stream per slot"},
+        "p50_ms": top["server"]["p50_ms"],
+        "p99_ms": top["server"]["p99_ms"],
+        "plain": top["plain"],
+        "throughput_vs_plain": round(top["server"]["throughput_rps"] / top["plain"]["throughput_rps"], 2),
+        "sweep": table,
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -486,6 +539,8 @@ def main():
         out = run_hea(args)
     elif args.workload == "ucc":
         out = run_ucc(args)
+    elif args.workload == "serve":
+        out = run_serve(args)
     else:
         out = run_brickwork(args)
         out["cpu_baseline"] = None if args.no_cpu_baseline else cpu_baseline_brickwork(

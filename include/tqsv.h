@@ -212,6 +212,12 @@ int tqsv_ham_create(int n, int nterms, const uint64_t* xmask, const uint64_t* zm
  * sum_k coeff_k <P_k> into *energy (host, may be NULL) and <psi|psi> into
  * *norm2 (host, may be NULL).  All accumulated in FP64; the energy is NOT
  * divided by the norm. */
+/* Replace the coefficients of a Hamiltonian (same terms, same order), e.g. a
+ * field strength that changes between requests; the term structure and the
+ * device plans are kept.  No counterpart in the reference (whose Pauli sums
+ * are plain lists, vqe.py:9-20). */
+int tqsv_ham_set_coeffs(tqsv_ham* ham, int nterms, const double* coeff, void* stream);
+
 int tqsv_ham_expval(tqsv_ham* ham, const void* psi, double* terms, double* energy,
                     double* norm2, void* stream);
 

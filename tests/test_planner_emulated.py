@@ -120,3 +120,62 @@ def test_long_rotation_chain_rescales():
     assert np.isfinite(got).all()
     np.testing.assert_allclose(got, want, atol=2e-4)
     assert abs(np.linalg.norm(got) - 1) < 1e-4
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_controlled_swaps_move_pending_phases(seed):
+    # antidiagonal unit-modulus gates (X, Y, X times phases), uncontrolled or
+    # with one control of either value, interleaved with pending diagonals
+    # (1-bit phases, sign and general 2-bit phases) and rotations, so that the
+    # pending factor is moved through the swaps in every configuration:
+    # control in the registers, on a thread bit or outside the tile
+    n = 13
+    rng = np.random.default_rng(500 + seed)
+    items = []
+    for _ in range(220):
+        r = rng.random()
+        q = int(rng.integers(n))
+        c = int(rng.integers(n - 1))
+        c = c if c < q else c + 1
+        if r < 0.4:
+            a, b = np.exp(1j * rng.uniform(0, 2 * np.pi, 2)) if rng.random() < 0.4 else (1, 1)
+            if rng.random() < 0.2:
+                a, b = -1j, 1j
+            kind = int(rng.integers(3))
+            cm = 0 if kind == 0 else 1 << (n - 1 - c)
+            cv = cm if kind == 1 else 0
+            items.append(("m1", q, np.array([[0, a], [b, 0]]), cm, cv))
+        elif r < 0.65:
+            d = [1, 1, 1, -1] if rng.random() < 0.5 else np.exp(1j * rng.uniform(0, 2 * np.pi, 4))
+            items.append(("m2", q, c, np.diag(d)))
+        elif r < 0.8:
+            items.append(("m1", q, np.diag([1, np.exp(1j * rng.uniform(0, 2 * np.pi))]), 0, 0))
+        else:
+            items.append(("m1", q, O.gate_ry(rng.uniform(0, 6)), 0, 0))
+    v = rand_state(rng, n)
+    want = dense_apply_items(n, items, v)
+    ops = ops_from_items(n, items)
+    for opts in ({}, {"max_cost": 100000}, {"tile_bits": 11, "max_cost": 6},
+                 {"tile_bits": 12, "min_low_bits": 2}):
+        got = nat.emulate(n, ops, v.astype(np.complex64), **opts)
+        np.testing.assert_allclose(got, want, atol=5e-5)
+
+
+def test_cx_ladder_ansatz_pass_count():
+    # RY/RZ layers with CX ladders: the CX steps are register renamings and
+    # the RZ phases travel through them, so a 26-qubit, 12-layer circuit
+    # needs few passes
+    from tyxonq_amd import serving as S
+
+    req = S.Request(0, 26, 12, "tfim", 1.0, np.random.default_rng(1).uniform(0, 6.28, 624))
+    info = nat.inspect(26, S.request_ops(req))
+    assert info["passes"] <= 34, info  # 46 with the CX as a general 2x2 step
+    small = S.Request(0, 12, 4, "tfim", 1.0, np.random.default_rng(2).uniform(0, 6.28, 96))
+    v = rand_state(np.random.default_rng(3), 12)
+    ops = S.request_ops(small)
+    want = v.copy()
+    for row in ops:
+        m = np.array(row["m"][:8:2] + 1j * row["m"][1:8:2]).reshape(2, 2)
+        item = ("m1", 11 - int(row["t"][0]), m, int(row["cmask"]), int(row["cval"]))
+        want = dense_apply_items(12, [item], want)
+    np.testing.assert_allclose(nat.emulate(12, ops, v.astype(np.complex64)), want, atol=5e-5)

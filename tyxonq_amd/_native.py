@@ -111,6 +111,7 @@ _SIGS = {
     "tqsv_plan_destroy": (None, [_P]),
     "tqsv_apply_fused": (_I, [_P, _I, _P, _I, ctypes.POINTER(PlanOpts), _P]),
     "tqsv_ham_create": (_I, [_I, _I, _P, _P, _P, ctypes.POINTER(_P)]),
+    "tqsv_ham_set_coeffs": (_I, [_P, _I, _DP, _P]),
     "tqsv_ham_expval": (_I, [_P, _P, _DP, _DP, _DP, _P]),
     "tqsv_ham_apply": (_I, [_P, _P, _P, _P]),
     "tqsv_ham_get_info": (_I, [_P, ctypes.POINTER(HamInfo)]),
@@ -427,6 +428,14 @@ class Ham:
     @property
     def nterms(self) -> int:
         return len(self.c)
+
+    def set_coeffs(self, coeff: Sequence[float]) -> None:
+        """New coefficients for the same terms (plans and device data kept)."""
+        c = np.ascontiguousarray(np.asarray(coeff, dtype=np.float64))
+        if c.shape != (self.nterms,):
+            raise ValueError(f"expected {self.nterms} coefficients, got {c.shape}")
+        _check(load().tqsv_ham_set_coeffs(self._h, self.nterms, c.ctypes.data_as(_DP), stream_ptr()))
+        self.c = c
 
     def expval(self, psi, per_term: bool = False):
         """Returns (energy, norm2) or (energy, norm2, terms)."""

@@ -395,6 +395,98 @@ __device__ __forceinline__ void scl_step(float2 (&a)[NSLOT], float2 (&b)[NSLOT],
   }
 }
 
+// ---- OP_CXP -------------------------------------------------------------
+// Controlled swap on slot bit K.  With the control a slot bit (CR < RB) or
+// absent (CR == CX_NONE) the swap is a renaming of registers; with the
+// control a thread bit (CR == CX_THREAD) each lane selects by its own test.
+template <int K, int CR, int CV>
+__device__ __forceinline__ void cxp_apply(float2 (&a)[NSLOT], bool ton) {
+#pragma unroll
+  for (int s = 0; s < NSLOT; ++s) {
+    if (s & (1 << K)) continue;
+    const int s1 = s | (1 << K);
+    if constexpr (CR < RB) {
+      if (((s >> CR) & 1) != CV) continue;
+    }
+    const float2 x0 = a[s], x1 = a[s1];
+    if constexpr (CR == (int)CX_THREAD) {
+      a[s] = ton ? x1 : x0;
+      a[s1] = ton ? x0 : x1;
+    } else {
+      a[s] = x1;
+      a[s1] = x0;
+    }
+  }
+}
+
+template <typename IDX>
+__device__ __forceinline__ bool cxp_thread_on(cw_t w, const PhaseGeo<IDX>& G) {
+  const IDX cm = ld_idx<IDX>(w + 4);
+  const IDX cv = ld_idx<IDX>(w + 6);
+  return (G.gb & cm) == cv;
+}
+
+// interpreter form: smask (bit s: the register part of the control holds
+// for slot s) is uniform, ton is the thread part.  Dual passes swap both
+// states per pair, so that each pair's selector dies at once.
+template <int K, bool DUAL>
+__device__ __forceinline__ void cxp_apply_rt(float2 (&a)[NSLOT], float2 (&b)[NSLOT], bool ton,
+                                             uint32_t smask) {
+#pragma unroll
+  for (int s = 0; s < NSLOT; ++s) {
+    if (s & (1 << K)) continue;
+    const int s1 = s | (1 << K);
+    const bool on = ((smask >> s) & 1u) != 0 && ton;
+    const float2 x0 = a[s], x1 = a[s1];
+    a[s] = on ? x1 : x0;
+    a[s1] = on ? x0 : x1;
+    if constexpr (DUAL) {
+      const float2 y0 = b[s], y1 = b[s1];
+      b[s] = on ? y1 : y0;
+      b[s1] = on ? y0 : y1;
+    }
+  }
+}
+
+template <int K, int MODE, typename IDX>
+__device__ __forceinline__ void cxp_dispatch(float2 (&a)[NSLOT], float2 (&b)[NSLOT], cw_t w,
+                                             const PhaseGeo<IDX>& G) {
+  const uint32_t smask = w[8];
+  const bool ton = w[2] == CX_THREAD ? cxp_thread_on<IDX>(w, G) : true;
+  cxp_apply_rt<K, MODE == MODE_DUAL>(a, b, ton, smask);
+}
+
+template <int MODE, typename IDX>
+__device__ __forceinline__ void cxp_switch(uint32_t k, float2 (&a)[NSLOT], float2 (&b)[NSLOT],
+                                           cw_t w, const PhaseGeo<IDX>& G) {
+#define TQSV_CXP(K)                                \
+  case K:                                          \
+    cxp_dispatch<K, MODE, IDX>(a, b, w, G);        \
+    touch_all(a);                                  \
+    if constexpr (MODE == MODE_DUAL) touch_all(b); \
+    TQSV_ARM_END("cxp", K);                        \
+    break;
+  switch (k) {
+    TQSV_CXP(0)
+    TQSV_CXP(1)
+    TQSV_CXP(2)
+    TQSV_CXP(3)
+    TQSV_CXP(4)
+    default: break;
+  }
+#undef TQSV_CXP
+}
+
+// generated-kernel form: slot, control slot and value fixed at compile time
+template <int K, int CR, int CV, int MODE, typename IDX>
+__device__ __forceinline__ void jcxp(float2 (&a)[NSLOT], float2 (&b)[NSLOT], cw_t w,
+                                     const PhaseGeo<IDX>& G) {
+  bool ton = true;
+  if constexpr (CR == (int)CX_THREAD) ton = cxp_thread_on<IDX>(w, G);
+  cxp_apply<K, CR, CV>(a, ton);
+  if constexpr (MODE == MODE_DUAL) cxp_apply<K, CR, CV>(b, ton);
+}
+
 // ---- OP_D2 --------------------------------------------------------------
 template <int MODE, typename IDX>
 __device__ __forceinline__ void d2_step(float2 (&a)[NSLOT], float2 (&b)[NSLOT], cw_t w,

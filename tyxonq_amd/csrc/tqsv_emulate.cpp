@@ -10,6 +10,7 @@
 #include <complex>
 #include <cstring>
 #include <memory>
+#include <utility>
 #include <vector>
 
 #include "tqsv_planner.h"
@@ -161,6 +162,21 @@ void run_steps(const uint32_t* w, uint32_t nsteps, cf* a, cf* b, bool dual, uint
         for (int z = 0; z < narr; ++z)
           for (int s = 0; s < NSLOT; ++s)
             arrs[z][s] *= cf(fw(w + TAB_OFF + 2 * s), fw(w + TAB_OFF + 2 * s + 1));
+        break;
+      }
+      case OP_CXP: {
+        // the encoding the interpreting kernel reads: the register part of
+        // the control as a slot mask, a thread-bit control by the global masks
+        const int K = (int)w[1];
+        const uint64_t cm = w64(w + 4), cv = w64(w + 6);
+        const bool ton = w[2] == CX_THREAD ? (gb & cm) == cv : true;
+        for (int z = 0; z < narr; ++z) {
+          cf* x = arrs[z];
+          for (int s = 0; s < NSLOT; ++s) {
+            if (s & (1 << K)) continue;
+            if (((w[8] >> s) & 1u) && ton) std::swap(x[s], x[s | (1 << K)]);
+          }
+        }
         break;
       }
       case OP_SCL: {

@@ -50,6 +50,8 @@ struct tqsv_ham {
   uint32_t* d_gstart = nullptr;
   uint64_t* d_tz = nullptr;
   float2* d_tc = nullptr;
+  std::vector<int> apply_order;  // term of each H|psi> coefficient slot
+  bool jit_ready = false;        // generated expectation kernels compiled
 
   ~tqsv_ham() {
     (void)hipFree(dprog);
@@ -304,34 +306,43 @@ int upload_expval(tqsv_ham* h) {
   return 0;
 }
 
+// coefficients of H|psi> in group order: coeff * i^k
+std::vector<float2> apply_coeffs(const tqsv_ham* h) {
+  std::vector<float2> tc;
+  tc.reserve(h->apply_order.size());
+  for (int t : h->apply_order) {
+    const int k = popc(h->x[t] & h->z[t]) & 3;
+    const double c = h->coeff[t];
+    float2 v;
+    switch (k) {
+      case 0: v = make_float2((float)c, 0.f); break;
+      case 1: v = make_float2(0.f, (float)c); break;
+      case 2: v = make_float2((float)-c, 0.f); break;
+      default: v = make_float2(0.f, (float)-c); break;
+    }
+    tc.push_back(v);
+  }
+  return tc;
+}
+
 int plan_apply(tqsv_ham* h) {
   std::map<uint64_t, std::vector<int>> groups;
   for (int t = 0; t < h->nterms; ++t) groups[h->x[t]].push_back(t);
   std::vector<uint64_t> gx, tzv;
   std::vector<uint32_t> gstart;
-  std::vector<float2> tc;
+  h->apply_order.clear();
   for (auto& kv : groups) {
     gx.push_back(kv.first);
     gstart.push_back((uint32_t)tzv.size());
     for (int t : kv.second) {
       tzv.push_back(h->z[t]);
-      const int k = popc(h->x[t] & h->z[t]) & 3;
-      // coeff * i^k
-      const double c = h->coeff[t];
-      float2 v;
-      switch (k) {
-        case 0: v = make_float2((float)c, 0.f); break;
-        case 1: v = make_float2(0.f, (float)c); break;
-        case 2: v = make_float2((float)-c, 0.f); break;
-        default: v = make_float2(0.f, (float)-c); break;
-      }
-      tc.push_back(v);
+      h->apply_order.push_back(t);
     }
   }
   gstart.push_back((uint32_t)tzv.size());
   h->ngroups = (uint32_t)gx.size();
   if (upload(&h->d_gx, gx) || upload(&h->d_gstart, gstart) || upload(&h->d_tz, tzv) ||
-      upload(&h->d_tc, tc))
+      upload(&h->d_tc, apply_coeffs(h)))
     return TQSV_EDEVICE;
   return 0;
 }
@@ -360,6 +371,11 @@ double term_value(const tqsv_ham* h, int t) {
 
 int run_expval(tqsv_ham* h, const float2* psi, hipStream_t st) {
   const int n = h->n;
+  if (h->jit && !h->jit_ready) {  // compiled on first use: H|psi> alone needs no kernels
+    const int rc = jit_prepare(h->passes, h->prog, n);
+    if (rc) return rc;
+    h->jit_ready = true;
+  }
   for (const PassPlan& pp : h->passes) {
     PassLaunch pl;
     pl.psi = const_cast<float2*>(psi);
@@ -421,7 +437,7 @@ int plan_build_forward(tqsv_plan* p, const tqsv_op* ops, int nops, hipStream_t s
   if (rc) return rc;
   merge_and_classify(g);
   const bool same = same_structure(g, p->gates) && !p->fsel.empty();
-  if (g.size() == p->gates.size())
+  if (p->fgen && g.size() == p->gates.size())
     for (size_t i = 0; i < g.size(); ++i) g[i].rot_form = p->gates[i].rot_form;
   std::vector<PassPlan> sel;
   if (same) {
@@ -438,7 +454,8 @@ int plan_build_forward(tqsv_plan* p, const tqsv_op* ops, int nops, hipStream_t s
   p->fsel.swap(sel);
   p->max_param = -1;
   for (const Gate& x : g) p->max_param = std::max(p->max_param, x.param);
-  if (jit_policy(p->jit_req, p->n, count_tile(passes), 64)) {
+  p->fgen = jit_policy(p->jit_req, p->n, count_tile(passes), 64) != 0;
+  if (p->fgen) {
     rc = jit_prepare(passes, prog, p->n);
     if (rc) return rc;
   }
@@ -478,7 +495,7 @@ int plan_build_adjoint(tqsv_plan* p, hipStream_t st) {
     r.push_back(dagger(g));
   }
   const bool same = same_structure(r, p->rgates) && !p->rsel.empty();
-  if (r.size() == p->rgates.size())
+  if (p->rgen && r.size() == p->rgates.size())
     for (size_t i = 0; i < r.size(); ++i) r[i].rot_form = p->rgates[i].rot_form;
   SchedOpts so = p->so;
   so.dual = true;
@@ -496,7 +513,8 @@ int plan_build_adjoint(tqsv_plan* p, hipStream_t st) {
   rc = emit_passes(r, so, sel, passes, prog, red_total);
   if (rc) return rc;
   p->rsel.swap(sel);
-  if (jit_policy(p->jit_req, p->n, count_tile(passes), 64)) {
+  p->rgen = jit_policy(p->jit_req, p->n, count_tile(passes), 64) != 0;
+  if (p->rgen) {
     rc = jit_prepare(passes, prog, p->n);
     if (rc) return rc;
   }
@@ -583,10 +601,6 @@ int tqsv_ham_create(int n, int nterms, const uint64_t* xmask, const uint64_t* zm
     }
     rc = upload_expval(h.get());
     if (rc) return rc;
-    if (h->jit) {
-      rc = jit_prepare(h->passes, h->prog, n);
-      if (rc) return rc;
-    }
     rc = plan_apply(h.get());
     if (rc) return rc;
     *out = h.release();
@@ -595,6 +609,22 @@ int tqsv_ham_create(int n, int nterms, const uint64_t* xmask, const uint64_t* zm
     return fail(TQSV_ENOMEM, "tqsv_ham_create: out of host memory");
   } catch (const std::exception& e) {
     return fail(TQSV_EINVAL, std::string("tqsv_ham_create: ") + e.what());
+  }
+}
+
+int tqsv_ham_set_coeffs(tqsv_ham* h, int nterms, const double* coeff, void* stream) {
+  try {
+    if (!h || (nterms > 0 && !coeff)) return fail(TQSV_EINVAL, "tqsv_ham_set_coeffs: NULL argument");
+    if (nterms != h->nterms) return fail(TQSV_EINVAL, "tqsv_ham_set_coeffs: term count differs");
+    for (int t = 0; t < nterms; ++t) h->coeff[t] = coeff[t];
+    if (h->apply_order.empty()) return 0;
+    const std::vector<float2> tc = apply_coeffs(h);
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    TQSV_HIP(hipMemcpyAsync(h->d_tc, tc.data(), tc.size() * sizeof(float2), hipMemcpyHostToDevice, st));
+    TQSV_HIP(hipStreamSynchronize(st));
+    return 0;
+  } catch (const std::bad_alloc&) {
+    return fail(TQSV_ENOMEM, "tqsv_ham_set_coeffs: out of host memory");
   }
 }
 

@@ -111,6 +111,10 @@ std::string gen_source(int tb, const uint32_t* pw, int n) {
             << ", MODE>(a, b, " << P << ", G);\n";
           break;
         case OP_SCL: o << "      scl_step<MODE>(a, b, " << P << ");\n"; break;
+        case OP_CXP:
+          o << "      jcxp<" << w[1] << ", " << w[2] << ", " << w[3] << ", MODE>(a, b, " << P
+            << ", G);\n";
+          break;
         case OP_PH:
           o << "      jph<" << w[1] << ", " << w[2] << ", MODE>(a, b, " << P << ", G);\n";
           break;

@@ -108,6 +108,7 @@ __global__ __launch_bounds__(1 << TB) void k_pass(float2* __restrict__ psi, floa
             case OP_TAB: tab_step<MODE>(a, b, w); break;
             case OP_THR: thr_step<MODE, IDX>(a, b, w, G); break;
             case OP_SCL: scl_step<MODE>(a, b, w); break;
+            case OP_CXP: cxp_switch<MODE, IDX>(w[1], a, b, w, G); break;
             case OP_U1: u1_switch<MODE, IDX>(w[1], a, b, w, G); break;
             case OP_U2: u2_switch<MODE, IDX>(w[1], w[2], a, b, w, G); break;
             case OP_D2: d2_step<MODE, IDX>(a, b, w, G); break;

@@ -47,10 +47,21 @@ bool unitary_like(const cd m[4]) {
   return std::abs(std::conj(m[0]) * m[1] + std::conj(m[2]) * m[3]) <= 1e-9 * r2;
 }
 
+// A fixed (unparameterised) gate with at most one control and an
+// antidiagonal matrix of unit-modulus entries: a controlled swap times a
+// diagonal.  The swap is a register renaming and the diagonal joins the
+// pending phase polynomial.
+bool perm_like(const Gate& g) {
+  if (g.kind != GK_MAT1 || g.param >= 0 || popc(g.cmask) > 1) return false;
+  if (!is0(g.m[0]) || !is0(g.m[3])) return false;
+  return std::abs(std::abs(g.m[1]) - 1.0) <= 1e-12 && std::abs(std::abs(g.m[2]) - 1.0) <= 1e-12;
+}
+
 // approximate VALU work per amplitude, for the per-pass budget
 int gate_cost(const Gate& g) {
   switch (g.kind) {
     case GK_MAT1:
+      if (perm_like(g)) return 1;
       if (g.cmask == 0 && unitary_like(g.m)) return 4;  // rotation + one-sided phase
       return (mat1_type(g) == U1_GENERAL ? 8 : 4) + (g.cmask ? 4 : 0);
     case GK_MAT2: return g.cmask ? 20 : 16;
@@ -291,6 +302,45 @@ struct Pend {
     lin[q] = cd(0, 0);
     lmask &= ~bitm(q);
   }
+  // Moving a permutation pi of the index bits through D: pi D = D' pi with
+  // D'(z) = D(pi(z)) (pi is its own inverse here).
+  // flip: z_t -> 1 - z_t (an X on bit t).
+  void flip(int t) {
+    if (lmask & bitm(t)) {
+      c0 = wrap(c0 + lin[t]);
+      lin[t] = wrap(-lin[t]);
+    }
+    for (uint64_t u = qm[t]; u; u &= u - 1) {
+      const int q = ctz(u);
+      cd& r = qref(t, q);
+      const cd v = r;
+      r = wrap(-v);
+      add_lin(q, v);
+    }
+  }
+  // cx: z_t -> z_t xor z_c = z_t + z_c - 2 z_t z_c (a CX with control c).
+  // lin_t z_t stays and adds lin_t z_c - 2 lin_t z_t z_c; quad_tc z_t z_c
+  // becomes quad_tc z_c - quad_tc z_t z_c; quad_tq z_t z_q (q != c) adds
+  // quad_tq z_c z_q and a cubic term -2 quad_tq z_t z_c z_q, which vanishes
+  // (exp(-2 i pi) = 1) only when quad_tq is a sign: cx_ok tests that.
+  bool cx_ok(int t, int c) const {
+    for (uint64_t u = qm[t] & ~bitm(c); u; u &= u - 1) {
+      const int q = ctz(u);
+      if (!is_sign(t < q ? quad[t * 64 + q] : quad[q * 64 + t])) return false;
+    }
+    return true;
+  }
+  void cx(int t, int c) {
+    const cd lt = (lmask & bitm(t)) ? lin[t] : cd(0, 0);
+    const cd qtc = (qm[t] & bitm(c)) ? qref(t, c) : cd(0, 0);
+    for (uint64_t u = qm[t] & ~bitm(c); u; u &= u - 1) {
+      const int q = ctz(u);
+      add_quad(c, q, qref(t, q));
+    }
+    add_lin(c, lt + qtc);
+    clear_quad(t, c);
+    add_quad(t, c, -2.0 * lt - qtc);
+  }
   // d[2*bit(a) + bit(b)]; a == b is a one-bit diagonal (d[0], d[3]).
   // Returns false when a factor is zero or not of unit modulus.
   bool add_diag2(int a, int b, const cd d[4]) {
@@ -525,6 +575,48 @@ struct PhaseEmitter {
     rescale();
   }
 
+  // a perm_like gate: its diagonal joins the pending factor, the factor moves
+  // through the swap, and the swap is emitted
+  void cx(const Gate& g) {
+    const int t = g.t0;
+    const int c = g.cmask ? ctz(g.cmask) : -1;
+    const int cv = c >= 0 ? (int)((g.cval >> c) & 1u) : 1;
+    // [[0, m1], [m2, 0]] = X diag(m2, m1)
+    if (!(is1(g.m[1]) && is1(g.m[2]))) {
+      cd d[4];
+      if (c < 0) {
+        d[0] = g.m[2];
+        d[1] = d[2] = cd(1, 0);
+        d[3] = g.m[1];
+        P.add_diag2(t, t, d);
+      } else {
+        for (int zc = 0; zc < 2; ++zc)
+          for (int zt = 0; zt < 2; ++zt)
+            d[2 * zc + zt] = zc == cv ? (zt ? g.m[1] : g.m[2]) : cd(1, 0);
+        P.add_diag2(c, t, d);
+      }
+    }
+    if (c >= 0 && !P.cx_ok(t, c)) extract(t);
+    if (cv == 0 || c < 0) P.flip(t);
+    if (c >= 0) P.cx(t, c);
+    const uint32_t creg =
+        c < 0 ? CX_NONE : (S.of_bit[c] >= 0 ? (uint32_t)S.of_bit[c] : CX_THREAD);
+    uint32_t smask = ~0u;
+    if (creg < (uint32_t)RB) {
+      smask = 0;
+      for (int s = 0; s < NSLOT; ++s)
+        if ((int)((s >> creg) & 1) == cv) smask |= 1u << s;
+    }
+    const size_t at = e.begin(OP_CXP);
+    e.u((uint32_t)S.of_bit[t]);
+    e.u(creg);
+    e.u((uint32_t)cv);
+    e.u64(g.cmask);
+    e.u64(g.cval);
+    e.u(smask);
+    e.end(at);
+  }
+
   // apply everything that is pending
   void flush() {
     regreg();
@@ -590,6 +682,10 @@ void emit_phase(std::vector<Gate>& gates, const std::vector<int>& sel,
     Gate& g = gates[gi];
     switch (g.kind) {
       case GK_MAT1: {
+        if (perm_like(g)) {
+          pe.cx(g);
+          break;
+        }
         RotDec d;
         if (g.cmask == 0 && decompose_u1(g.m, d, g.rot_form)) {
           g.rot_form = d.mode & 1;

@@ -41,9 +41,11 @@ struct Gate {
   double gcoef = 0.0;
   int red_slot = -1;
   double red_scale = 1.0;  // true <lambda|P|psi> = red_scale * (value read from the sweep)
-  // rotation form (tan 0 / cot 1) chosen when the gate was last emitted; kept
-  // across parameter updates while |t| <= 4 so that the program structure (and
-  // any compiled kernel) does not flip when an angle moves across |t| = 1
+  // rotation form (tan 0 / cot 1) chosen when the gate was last emitted; for
+  // plans on generated kernels it is kept across parameter updates while
+  // |t| <= 4, so that the compiled code does not change when an angle moves
+  // across |t| = 1 (other plans choose by |t| <= 1 alone, so that results do
+  // not depend on earlier parameters)
   int rot_form = -1;
   int src = -1;
 };
@@ -128,6 +130,7 @@ struct tqsv_plan {
   int n = 0;
   int nops = 0;
   int jit_req = 0;  // requested: 1 generated kernels, -1 precompiled, 0 automatic
+  bool fgen = false, rgen = false;  // the last build ran generated kernels (forward, reverse)
   tqsv::SchedOpts so;
   std::vector<tqsv::Gate> gates;          // forward gates after merge
   std::vector<tqsv::PassPlan> fsel;       // pass/phase selection of the forward gates

@@ -97,7 +97,18 @@ enum : uint32_t {
   OP_RED = 10,
   // real scale: a[s] *= f for all s.  [1]=f (float)
   OP_SCL = 11,
+  // controlled swap on slot bit k: a[s] <-> a[s | 1<<k] for the slots s (bit
+  // k clear) whose control holds.  [1]=k [2]=creg: the control's slot bit
+  // (0..RB-1), CX_THREAD when the control is a thread bit (tested on gb),
+  // CX_NONE without control [3]=control value (0/1) [4..6)=cmask (lo,hi)
+  // [6..8)=cval (lo,hi), both on global index bits [8]=smask: bit s set for
+  // the slots whose register bits pass the control (all slots unless creg is
+  // a slot bit)
+  OP_CXP = 12,
 };
+
+constexpr uint32_t CX_THREAD = RB;
+constexpr uint32_t CX_NONE = RB + 1;
 
 // OP_U1 types
 constexpr uint32_t U1_GENERAL = 0;

@@ -1,0 +1,286 @@
+"""Energy+gradient serving with continuous batching on one GPU.
+
+Requests (qubit count, hardware-efficient ansatz depth and angles, TFIM or
+Heisenberg Hamiltonian) arrive over time.  The server admits a request as
+soon as it has arrived, a slot is free and its adjoint footprint (2^n x 24 B:
+the state, lambda and headroom) fits the memory budget; every admitted
+request runs on a worker with its own HIP stream and resident device states,
+so requests that arrive while others are on the GPU overlap with them.  When
+one finishes, its slot and memory are released and the next request is
+admitted.  `plain_loop` runs the same requests one at a time in arrival order
+on one stream, for comparison.
+
+Each request's result depends only on the request: the reductions have a fixed
+order, so a request computes the same energy and gradient alone and in any mix.
+"""
+from __future__ import annotations
+
+import math
+import threading
+import time
+from concurrent.futures import ThreadPoolExecutor
+from dataclasses import dataclass
+from functools import lru_cache
+from typing import Dict, List, Tuple
+
+import numpy as np
+
+from . import _native as nat
+from .pauli import PauliSum
+
+Terms = List[Tuple[float, List[Tuple[str, int]]]]
+
+
+@dataclass
+class Request:
+    rid: int
+    n: int
+    depth: int
+    ham: str            # "tfim" or "heisenberg"
+    g: float
+    params: np.ndarray  # 2 * n * depth angles: RY then RZ per layer
+    arrival: float = 0.0
+
+
+@dataclass
+class Result:
+    rid: int
+    energy: float
+    grad: np.ndarray
+    arrival: float
+    start: float
+    done: float
+
+    @property
+    def latency(self) -> float:
+        return self.done - self.arrival
+
+
+def make_requests(count: int, rate: float, seed: int = 1234, nmin: int = 10, nmax: int = 26,
+                  dmin: int = 2, dmax: int = 16) -> List[Request]:
+    """Poisson arrivals at `rate` per second; per request n ~ U{nmin..nmax},
+    depth ~ U{dmin..dmax}, H in {TFIM (open chain), Heisenberg (periodic)}
+    with g ~ U[0.5, 1.5], angles ~ U[0, 2pi).  Draw order per request: gap,
+    n, depth, Hamiltonian kind, g, angles."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for k in range(count):
+        t += rng.exponential(1.0 / rate)
+        n = int(rng.integers(nmin, nmax + 1))
+        d = int(rng.integers(dmin, dmax + 1))
+        kind = "tfim" if rng.random() < 0.5 else "heisenberg"
+        g = float(rng.uniform(0.5, 1.5))
+        th = rng.uniform(0.0, 2.0 * math.pi, size=2 * n * d)
+        out.append(Request(k, n, d, kind, g, th, t))
+    return out
+
+
+def request_terms(req: Request) -> Terms:
+    n, g = req.n, req.g
+    if req.ham == "tfim":
+        t: Terms = [(-1.0, [("Z", i), ("Z", i + 1)]) for i in range(n - 1)]
+        return t + [(-g, [("X", i)]) for i in range(n)]
+    t = []
+    for i in range(n):
+        j = (i + 1) % n
+        for P in "XYZ":
+            t.append((1.0, [(P, i), (P, j)]))
+    return t + [(g, [("Z", i)]) for i in range(n)]
+
+
+@lru_cache(maxsize=None)
+def _ops_template(n: int, depth: int):
+    """Op rows of the ansatz with zero angles, and the row indices of the RY
+    and RZ rotations (parameter k is row ry[k] or rz[k])."""
+    rows, ry, rz = [], [], []
+    X = [0.0] * 32
+    X[2] = X[4] = 1.0
+    k = 0
+    for _ in range(depth):
+        for q in range(n):
+            b = 1 << (n - 1 - q)
+            ry.append(len(rows))
+            rows.append((nat.OP_MAT1, (n - 1 - q, -1), k, 0, 0, 0, 0, 0.0, b, b, 1.0, [0.0] * 32))
+            k += 1
+        for q in range(n):
+            b = 1 << (n - 1 - q)
+            rz.append(len(rows))
+            rows.append((nat.OP_MAT1, (n - 1 - q, -1), k, 0, 0, 0, 0, 0.0, 0, b, 1.0, [0.0] * 32))
+            k += 1
+        for q in range(n - 1):
+            cb = 1 << (n - 1 - q)
+            rows.append((nat.OP_MAT1, (n - 2 - q, -1), -1, cb, cb, 0, 0, 0.0, 0, 0, 0.0, X))
+    return np.array(rows, dtype=nat.OP_DTYPE), np.array(ry), np.array(rz)
+
+
+def request_ops(req: Request) -> np.ndarray:
+    """depth x [RY(theta) on every qubit, RZ(theta) on every qubit, CX ladder
+    q -> q+1], as bit-level ops with one parameter per rotation."""
+    tmpl, ry, rz = _ops_template(req.n, req.depth)
+    ops = tmpl.copy()
+    th = np.asarray(req.params, dtype=np.float64).reshape(req.depth, 2, req.n)
+    c, s = np.cos(th / 2), np.sin(th / 2)
+    m = ops["m"]
+    # RY = [[c, -s], [s, c]];  RZ = diag(c - i s, c + i s)
+    m[ry, 0] = c[:, 0].ravel()
+    m[ry, 2] = -s[:, 0].ravel()
+    m[ry, 4] = s[:, 0].ravel()
+    m[ry, 6] = c[:, 0].ravel()
+    m[rz, 0] = c[:, 1].ravel()
+    m[rz, 1] = -s[:, 1].ravel()
+    m[rz, 6] = c[:, 1].ravel()
+    m[rz, 7] = s[:, 1].ravel()
+    return ops
+
+
+def request_coeffs(req: Request) -> np.ndarray:
+    """Coefficients of request_terms(req) (the terms depend on n and kind only)."""
+    n, g = req.n, req.g
+    if req.ham == "tfim":
+        return np.concatenate([np.full(n - 1, -1.0), np.full(n, -g)])
+    return np.concatenate([np.ones(3 * n), np.full(n, g)])
+
+
+class Cache:
+    """Resources reused across requests of one worker: plans per (n, depth)
+    (a new request only re-emits the programs) and Hamiltonians per
+    (n, kind) (a new request only replaces the coefficients)."""
+
+    def __init__(self):
+        self.plans: Dict[Tuple[int, int], "nat.Plan"] = {}
+        self.hams: Dict[Tuple[int, str], "nat.Ham"] = {}
+
+
+class _Worker:
+    """Per-thread resources: a HIP stream and a Cache."""
+
+    def __init__(self):
+        import torch
+
+        self.stream = torch.cuda.Stream()
+        self.cache = Cache()
+
+
+def execute(req: Request, cache: Cache) -> Tuple[float, np.ndarray]:
+    """Energy and gradient of one request on the current stream."""
+    import torch
+
+    ops = request_ops(req)
+    key = (req.n, req.depth)
+    plan = cache.plans.get(key)
+    if plan is None:
+        plan = nat.Plan(req.n, ops, jit=-1)
+        cache.plans[key] = plan
+    else:
+        plan.update(ops)
+    hk = (req.n, req.ham)
+    ham = cache.hams.get(hk)
+    if ham is None:
+        x, z, c = PauliSum(req.n, request_terms(req)).to_masks()
+        ham = nat.Ham(req.n, x, z, c)
+        cache.hams[hk] = ham
+    else:
+        ham.set_coeffs(request_coeffs(req))
+    psi = nat.new_state(req.n, 0)
+    lam = torch.empty_like(psi)
+    e, g = plan.adjoint(ham, psi, lam, len(req.params))
+    return float(e), g
+
+
+class Server:
+    """Continuous-batching server: up to `slots` requests in flight, admitted
+    in arrival order while their footprint fits `mem_budget` bytes."""
+
+    def __init__(self, slots: int = 8, mem_budget: float = 64e9):
+        self.slots = int(slots)
+        self.mem_budget = float(mem_budget)
+        self._local = threading.local()
+
+    @staticmethod
+    def footprint(req: Request) -> float:
+        return float(1 << req.n) * 24.0
+
+    def _run_one(self, req: Request) -> Tuple[float, float, np.ndarray]:
+        import torch
+
+        w = getattr(self._local, "w", None)
+        if w is None:
+            w = _Worker()
+            self._local.w = w
+        start = time.perf_counter()
+        with torch.cuda.stream(w.stream):
+            e, g = execute(req, w.cache)
+        return start, e, g
+
+    def run(self, reqs: List[Request], realtime: bool = True) -> List[Result]:
+        """Serve `reqs` (arrival times relative to the call) and return results
+        in request order."""
+        pending = sorted(reqs, key=lambda r: r.arrival)
+        results: Dict[int, Result] = {}
+        t0 = time.perf_counter()
+        lock = threading.Condition()
+        state = {"mem": 0.0, "busy": 0}
+
+        errors: List[BaseException] = []
+
+        def done_cb(req, fut, t_arr):
+            try:
+                start, e, g = fut.result()
+            except BaseException as exc:  # recorded and re-raised after the run
+                errors.append(exc)
+                start, e, g = t_arr, float("nan"), np.zeros(0)
+            with lock:
+                results[req.rid] = Result(req.rid, e, g, t_arr, start, time.perf_counter())
+                state["mem"] -= self.footprint(req)
+                state["busy"] -= 1
+                lock.notify_all()
+
+        with ThreadPoolExecutor(max_workers=self.slots) as ex:
+            i = 0
+            while i < len(pending):
+                req = pending[i]
+                t_arr = t0 + req.arrival if realtime else t0
+                now = time.perf_counter()
+                if realtime and now < t_arr:
+                    time.sleep(min(t_arr - now, 0.002))
+                    continue
+                with lock:
+                    while state["busy"] >= self.slots or (
+                            state["busy"] > 0 and state["mem"] + self.footprint(req) > self.mem_budget):
+                        lock.wait()
+                    state["busy"] += 1
+                    state["mem"] += self.footprint(req)
+                fut = ex.submit(self._run_one, req)
+                fut.add_done_callback(lambda f, r=req, ta=t_arr: done_cb(r, f, ta))
+                i += 1
+        if errors:
+            raise errors[0]
+        return [results[r.rid] for r in reqs]
+
+
+def plain_loop(reqs: List[Request], realtime: bool = True) -> List[Result]:
+    """The same requests one at a time, in arrival order, on one stream."""
+    import torch
+
+    cache = Cache()
+    out: Dict[int, Result] = {}
+    t0 = time.perf_counter()
+    for req in sorted(reqs, key=lambda r: r.arrival):
+        t_arr = t0 + req.arrival if realtime else t0
+        now = time.perf_counter()
+        if realtime and now < t_arr:
+            time.sleep(t_arr - now)
+        start = time.perf_counter()
+        e, g = execute(req, cache)
+        torch.cuda.current_stream().synchronize()
+        out[req.rid] = Result(req.rid, e, g, t_arr, start, time.perf_counter())
+    return [out[r.rid] for r in reqs]
+
+
+def summarize(results: List[Result]) -> Dict[str, float]:
+    lat = np.array([r.latency for r in results])
+    span = max(r.done for r in results) - min(r.arrival for r in results)
+    return {"requests": len(results), "throughput_rps": round(len(results) / span, 2),
+            "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
+            "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3)}
