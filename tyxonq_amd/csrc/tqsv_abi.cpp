@@ -239,7 +239,7 @@ int tqsv_plan_create(int n, const tqsv_op* ops, int nops, const tqsv_plan_opts* 
     if (opts) {
       p->so.tile_bits = opts->tile_bits > 0 ? opts->tile_bits : 13;
       p->so.min_low = opts->min_low_bits > 0 ? opts->min_low_bits : 5;
-      p->so.max_cost = opts->max_cost > 0 ? opts->max_cost : 96;
+      p->so.max_cost = opts->max_cost > 0 ? opts->max_cost : kDefaultMaxCost;
       p->so.fuse = opts->fuse != 0;
     }
     p->jit_req = opts ? opts->jit : 0;
@@ -297,7 +297,7 @@ int tqsv_plan_inspect(int n, const tqsv_op* ops, int nops, const tqsv_plan_opts*
     if (opts) {
       p.so.tile_bits = opts->tile_bits > 0 ? opts->tile_bits : 13;
       p.so.min_low = opts->min_low_bits > 0 ? opts->min_low_bits : 5;
-      p.so.max_cost = opts->max_cost > 0 ? opts->max_cost : 96;
+      p.so.max_cost = opts->max_cost > 0 ? opts->max_cost : kDefaultMaxCost;
       p.so.fuse = opts->fuse != 0;
     }
     std::vector<Gate> g;

@@ -57,6 +57,11 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
   return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 __device__ __forceinline__ float2 cneg(float2 a) { return make_float2(-a.x, -a.y); }
+// per-component select: a select of whole float2 values can become a select
+// of their addresses, which moves the operands to scratch memory
+__device__ __forceinline__ float2 sel2(bool c, float2 p, float2 q) {
+  return make_float2(c ? p.x : q.x, c ? p.y : q.y);
+}
 
 // per-phase geometry of one thread
 template <typename IDX>
@@ -159,8 +164,8 @@ __device__ __forceinline__ void u1_apply(float2 (&a)[NSLOT], const M2f& u, const
     float2 y0, y1;
     mul2<TYPE>(u, a[s], a[s1], y0, y1);
     if constexpr (CTRL) {
-      a[s] = ton ? y0 : a[s];
-      a[s1] = ton ? y1 : a[s1];
+      a[s] = sel2(ton, y0, a[s]);
+      a[s1] = sel2(ton, y1, a[s1]);
     } else {
       a[s] = y0;
       a[s1] = y1;
@@ -220,7 +225,7 @@ __device__ __forceinline__ void u2_apply(float2 (&a)[NSLOT], cw_t m, const Phase
       y[r] = make_float2(yr, yi);
     }
 #pragma unroll
-    for (int c = 0; c < 4; ++c) a[idx[c]] = (!CTRL || ton) ? y[c] : x[c];
+    for (int c = 0; c < 4; ++c) a[idx[c]] = CTRL ? sel2(ton, y[c], x[c]) : y[c];
   }
 }
 
@@ -320,7 +325,7 @@ __device__ __forceinline__ void ph_step(float2 (&a)[NSLOT], float2 (&b)[NSLOT], 
   for (uint32_t i = 0; i < nent; ++i, e += OPH_ENT) {
     const float2 v = cmul(u, make_float2(fl(e + 1), fl(e + 2)));
     const bool on = ((G.gb >> e[0]) & 1u) != 0;
-    u = on ? v : u;
+    u = sel2(on, v, u);
   }
 #define TQSV_PH(K)                                       \
   case K:                                               \
@@ -376,7 +381,7 @@ __device__ __forceinline__ void thr_step(float2 (&a)[NSLOT], float2 (&b)[NSLOT],
   for (uint32_t i = 0; i < nent; ++i, e += THR_ENT) {
     const float2 v = cmul(F, make_float2(fl(e + 2), fl(e + 3)));
     const bool on = (((G.gb >> e[0]) & (G.gb >> e[1])) & 1u) != 0;
-    F = on ? v : F;
+    F = sel2(on, v, F);
   }
 #pragma unroll
   for (int s = 0; s < NSLOT; ++s) {
@@ -410,8 +415,8 @@ __device__ __forceinline__ void cxp_apply(float2 (&a)[NSLOT], bool ton) {
     }
     const float2 x0 = a[s], x1 = a[s1];
     if constexpr (CR == (int)CX_THREAD) {
-      a[s] = ton ? x1 : x0;
-      a[s1] = ton ? x0 : x1;
+      a[s] = sel2(ton, x1, x0);
+      a[s1] = sel2(ton, x0, x1);
     } else {
       a[s] = x1;
       a[s1] = x0;
@@ -428,7 +433,9 @@ __device__ __forceinline__ bool cxp_thread_on(cw_t w, const PhaseGeo<IDX>& G) {
 
 // interpreter form: smask (bit s: the register part of the control holds
 // for slot s) is uniform, ton is the thread part.  Dual passes swap both
-// states per pair, so that each pair's selector dies at once.
+// states per pair, so that each pair's selector dies at once.  (Arms per
+// control slot would not be cheaper here: every arm must hand the slots over
+// in the same registers, so a renaming turns into moves at the join.)
 template <int K, bool DUAL>
 __device__ __forceinline__ void cxp_apply_rt(float2 (&a)[NSLOT], float2 (&b)[NSLOT], bool ton,
                                              uint32_t smask) {
@@ -438,12 +445,12 @@ __device__ __forceinline__ void cxp_apply_rt(float2 (&a)[NSLOT], float2 (&b)[NSL
     const int s1 = s | (1 << K);
     const bool on = ((smask >> s) & 1u) != 0 && ton;
     const float2 x0 = a[s], x1 = a[s1];
-    a[s] = on ? x1 : x0;
-    a[s1] = on ? x0 : x1;
+    a[s] = sel2(on, x1, x0);
+    a[s1] = sel2(on, x0, x1);
     if constexpr (DUAL) {
       const float2 y0 = b[s], y1 = b[s1];
-      b[s] = on ? y1 : y0;
-      b[s1] = on ? y0 : y1;
+      b[s] = sel2(on, y1, y0);
+      b[s1] = sel2(on, y0, y1);
     }
   }
 }

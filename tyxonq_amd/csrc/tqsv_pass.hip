@@ -18,7 +18,7 @@ size_t pass_lds_bytes(int tb, int mode, uint32_t nred) {
 }
 
 bool pass_variant_exists(int tb, int mode, bool idx64) {
-  if (idx64) return (mode == MODE_GATE && (tb == 8 || tb == 9)) || (mode == MODE_RED && tb == 8);
+  if (idx64) return (mode == MODE_GATE || mode == MODE_RED) && tb == 8;
   switch (mode) {
     case MODE_GATE: return tb >= 6 && tb <= 9;
     case MODE_RED: return tb >= 6 && tb <= 8;
@@ -30,7 +30,6 @@ bool pass_variant_exists(int tb, int mode, bool idx64) {
 hipError_t launch_pass(int tb, int mode, bool idx64, const PassLaunch& pl, hipStream_t st) {
   if (idx64) {
     if (mode == MODE_GATE && tb == 8) return launch_one<8, MODE_GATE, uint64_t>(pl, st);
-    if (mode == MODE_GATE && tb == 9) return launch_one<9, MODE_GATE, uint64_t>(pl, st);
     if (mode == MODE_RED && tb == 8) return launch_one<8, MODE_RED, uint64_t>(pl, st);
     return hipErrorInvalidValue;
   }

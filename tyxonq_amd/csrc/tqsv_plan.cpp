@@ -904,7 +904,7 @@ int effective_tile_bits(int n, int requested, bool dual) {
   if (T < 11) T = 11;
   if (T > 14) T = 14;
   if (dual && T > 13) T = 13;
-  if (n > 32 && T < 13) T = 13;
+  if (n > 32) T = 13;  // 64-bit index kernels exist for 2^13-amplitude tiles only
   if (n < T) T = n;
   return T;
 }
@@ -920,7 +920,7 @@ int select_passes(std::vector<Gate>& gates, const SchedOpts& so, std::vector<Pas
   if (Lmin < T - MAXH) Lmin = T - MAXH;  // at most MAXH high bits fit the pass header
   if (Lmin < 1) Lmin = 1;
   const uint64_t full = lowmask(n);
-  const int budget0 = so.max_cost > 0 ? so.max_cost : 96;
+  const int budget0 = so.max_cost > 0 ? so.max_cost : kDefaultMaxCost;
   // reduction slots per pass: the LDS beyond the tile(s) holds one double per
   // slot and wave
   const size_t lds_tiles = pass_lds_bytes(TBv, mode, 0);

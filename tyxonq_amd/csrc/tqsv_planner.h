@@ -77,11 +77,17 @@ struct PassPlan {
   void* jit_fn = nullptr;  // generated kernel (hipFunction_t), when the plan uses them
 };
 
+// Default per-pass budget (approximate VALU ops per amplitude).  Passes are
+// bound by VALU issue as much as by HBM; measured on 18-26 qubit RY/RZ/CX
+// adjoint plans, budgets from 96 up to about 400 cut the time (fewer passes,
+// less per-pass overhead) and larger ones change nothing.
+constexpr int kDefaultMaxCost = 400;
+
 struct SchedOpts {
   int n = 0;
   int tile_bits = 13;
   int min_low = 5;
-  int max_cost = 96;
+  int max_cost = kDefaultMaxCost;
   bool fuse = true;
   bool dual = false;
   bool tile_search = false;  // choose tile bits by admitted work (else in gate order)
