@@ -46,7 +46,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="brickwork",
-                    choices=["brickwork", "adjoint", "hea", "ucc", "serve"])
+                    choices=["brickwork", "adjoint", "hea", "ucc", "serve", "sample"])
     ap.add_argument("--n", type=int, default=None)
     ap.add_argument("--layers", type=int, default=None)
     ap.add_argument("--terms", type=int, default=500)
@@ -64,6 +64,7 @@ def parse():
                     help="serve workload: offered request rates (req/s)")
     ap.add_argument("--requests", type=int, default=120, help="serve workload: requests per rate")
     ap.add_argument("--slots", type=int, default=8, help="serve workload: requests in flight")
+    ap.add_argument("--shots", type=int, default=1_000_000, help="sample workload: shots per call")
     ap.add_argument("--shift-sample", type=int, default=8,
                     help="adjoint workload: parameters timed on the parameter-shift path")
     return ap.parse_args()
@@ -550,6 +551,78 @@ def run_serve(args) -> dict:
     }
 
 
+# ---- shot sampling (counts path of engine.run) ------------------------------------
+def run_sample(args) -> dict:
+    """Counts of `shots` draws from a 30-qubit state (8 GiB, random amplitudes)
+    with the GPU sampler; the state is resident before the timed calls."""
+    import torch
+
+    from tyxonq_amd import _native as nat
+
+    n = args.n or 30
+    shots = int(args.shots)
+    dev = nat.device()
+    g = torch.Generator(device=dev).manual_seed(args.seed)
+    psi = torch.complex(torch.randn(1 << n, device=dev, generator=g),
+                        torch.randn(1 << n, device=dev, generator=g)).to(torch.complex64)
+    smp = nat.Sampler(n, shots)
+    for w in range(args.warmup):
+        smp.sample(psi, shots, seed=w)
+    _sync()
+    s0, s1 = _events()
+    s0.record()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        idx, cnt = smp.sample(psi, shots, seed=1000 + k)
+    s1.record()
+    _sync()
+    wall = (time.perf_counter() - t0) / args.steps
+    gpu_ms = s0.elapsed_time(s1) / args.steps
+    assert int(cnt.sum()) == shots
+    # algorithmic bytes: two reads of the state (block sums, then the blocks
+    # that received shots: all of them at this shot count)
+    nbytes = 2 * 8 * (1 << n)
+    achieved = nbytes / (gpu_ms * 1e-3) / 1e9
+    return {
+        "metric": METRIC,
+        "value": round(shots / wall, 1),
+        "unit": "shots/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded random complex64 amplitudes)",
+        "config": {"workload": f"sample {shots} shots from a {n}-qubit state", "qubits": n,
+                   "shots": shots, "unique_states": int(idx.size), "parallelism": "1 GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0,
+                     "unit": "GB/s", "frac": round(achieved / 8000.0, 4), "traffic": None,
+                     "kernel": "sampler (block sums + per-block resolve)",
+                     "bytes_per_launch": nbytes, "avg_launch_ms": round(gpu_ms, 4)},
+        "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline_sample(n, shots),
+    }
+
+
+def cpu_baseline_sample(n: int, shots: int) -> dict:
+    """The reference's host path (|psi|^2, choice, bincount) on a 26-qubit
+    state, scaled by 2^(n-26) (the work is linear in the state size)."""
+    from oracle import sampling as OS
+
+    m = min(n, 26)
+    rng = np.random.default_rng(0)
+    v = (rng.normal(size=1 << m) + 1j * rng.normal(size=1 << m)).astype(np.complex64)
+    t0 = time.perf_counter()
+    OS.reference_counts(v, shots, seed=1)
+    dt = (time.perf_counter() - t0) * (1 << (n - m))
+    return {"value": round(shots / dt, 1), "unit": "shots/s", "cores": 1, "kind": "port",
+            "sample": f"numpy |psi|^2 + Generator.choice + bincount (engine.py:377-465) on a "
+                      f"{m}-qubit state with {shots} shots: {dt / (1 << (n - m)):.2f} s, "
+                      f"scaled x{1 << (n - m)} to {n} qubits"}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -565,6 +638,8 @@ def main():
         out = run_ucc(args)
     elif args.workload == "serve":
         out = run_serve(args)
+    elif args.workload == "sample":
+        out = run_sample(args)
     else:
         out = run_brickwork(args)
         out["cpu_baseline"] = None if args.no_cpu_baseline else cpu_baseline_brickwork(

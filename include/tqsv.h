@@ -242,6 +242,24 @@ int tqsv_expval(const void* psi, int n, int nterms, const uint64_t* xmask,
                 const uint64_t* zmask, const double* coeff, double* terms, double* energy,
                 double* norm2, void* stream);
 
+/* ---- shot sampling ------------------------------------------------------- */
+
+/* A sampler for n-qubit states: owns the device scratch for up to max_shots
+ * shots per call (about 50 B per shot plus 40 B per 4096 amplitudes). */
+typedef struct tqsv_sampler tqsv_sampler;
+int tqsv_sampler_create(int n, int64_t max_shots, tqsv_sampler** out);
+void tqsv_sampler_destroy(tqsv_sampler* sampler);
+
+/* Draw `shots` basis states from |psi|^2 / <psi|psi> (inverse CDF, FP64
+ * cumulative sums, Philox4x64-10 stream keyed by `seed`) and return the
+ * distinct states with their counts in increasing index order:
+ * index[0..*nunique), count[0..*nunique) (host arrays of at least
+ * min(shots, 2^n) entries).  Synchronises the stream.  Replaces the counts
+ * path of StatevectorEngine.run (devices/simulators/statevector/engine.py:
+ * 377-465: |psi|^2 on the host, numpy multinomial, bincount). */
+int tqsv_sampler_run(tqsv_sampler* sampler, const void* psi, int64_t shots, uint64_t seed,
+                     uint64_t* index, int64_t* count, int64_t* nunique, void* stream);
+
 /* ---- adjoint differentiation -------------------------------------------- */
 
 /* Energy and gradient of E(theta) = <psi(theta)|H|psi(theta)> by one forward

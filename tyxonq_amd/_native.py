@@ -10,7 +10,7 @@ from __future__ import annotations
 import ctypes
 import os
 from pathlib import Path
-from typing import Optional, Sequence
+from typing import Dict, Optional, Sequence
 
 import numpy as np
 
@@ -119,6 +119,10 @@ _SIGS = {
     "tqsv_expval": (_I, [_P, _I, _I, _P, _P, _P, _DP, _DP, _DP, _P]),
     "tqsv_adjoint": (_I, [_P, _P, _P, _P, _I, _DP, _DP, _P]),
     "tqsv_vjp": (_I, [_P, _P, _P, _I, _DP, _P]),
+    "tqsv_sampler_create": (_I, [_I, ctypes.c_int64, ctypes.POINTER(_P)]),
+    "tqsv_sampler_run": (_I, [_P, _P, ctypes.c_int64, _U64, _P, _P,
+                              ctypes.POINTER(ctypes.c_int64), _P]),
+    "tqsv_sampler_destroy": (None, [_P]),
 }
 
 
@@ -480,3 +484,58 @@ def version() -> int:
 
 if os.environ.get("TQSV_EAGER_LOAD"):
     load()
+
+
+class Sampler:
+    """Shot sampler for n-qubit states (owns its device scratch)."""
+
+    def __init__(self, n: int, max_shots: int):
+        load()
+        self.n = int(n)
+        self.max_shots = int(max_shots)
+        self._h = ctypes.c_void_p()
+        _check(load().tqsv_sampler_create(self.n, self.max_shots, ctypes.byref(self._h)))
+
+    def sample(self, psi, shots: int, seed: int = 0):
+        """(index, count) arrays of the distinct sampled basis states, index
+        ascending (uint64 / int64)."""
+        if _nqubits(psi) != self.n:
+            raise ValueError(f"sampler is for {self.n} qubits, state has {_nqubits(psi)}")
+        shots = int(shots)
+        cap = max(1, min(shots, 1 << self.n))
+        idx = np.empty(cap, dtype=np.uint64)
+        cnt = np.empty(cap, dtype=np.int64)
+        nu = ctypes.c_int64(0)
+        _check(load().tqsv_sampler_run(self._h, _ptr(psi), shots, int(seed) & ((1 << 64) - 1),
+                                       idx.ctypes.data, cnt.ctypes.data, ctypes.byref(nu),
+                                       stream_ptr()))
+        k = int(nu.value)
+        return idx[:k].copy(), cnt[:k].copy()
+
+    def close(self) -> None:
+        if self._h:
+            load().tqsv_sampler_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_samplers: Dict[int, "Sampler"] = {}
+
+
+def sample_counts(psi, shots: int, seed: int = 0):
+    """Sample `shots` basis states of psi on the GPU (a cached sampler per
+    qubit count, grown to the largest shot count seen)."""
+    n = _nqubits(psi)
+    s = _samplers.get(n)
+    if s is None or s.max_shots < int(shots):
+        if s is not None:
+            s.close()
+        s = Sampler(n, max(int(shots), 1024))
+        _samplers[n] = s
+    return s.sample(psi, shots, seed)
+

@@ -505,9 +505,23 @@ class StatevectorEngine:
                 "metadata": {"shots": shots, "backend": self.backend.name}}
 
     def _sample(self, psi, n: int, shots: int, kwargs: Dict[str, Any]) -> Dict[str, int]:
-        """Counts over the computational basis (engine.py:377-465): |psi|^2,
-        optional readout/depolarizing mixing, multinomial sampling, big-endian
-        bitstrings."""
+        """Counts over the computational basis (engine.py:377-465) as big-endian
+        bitstrings.  Without noise the shots are drawn on the GPU (inverse CDF
+        of |psi|^2, Philox stream keyed by ``seed``) and only the distinct
+        states come back.  Readout or depolarizing mixing of the probabilities
+        (engine.py:387-408) is applied to |psi|^2 on the host before a
+        multinomial draw, as in the reference."""
+        if not bool(kwargs.get("use_noise", False)):
+            seed = kwargs.get("seed")
+            if seed is None:
+                seed = int(np.random.SeedSequence().generate_state(1, dtype=np.uint64)[0])
+            try:
+                idx, cnt = nat.sample_counts(psi, shots, int(seed))
+                return {format(int(i), f"0{n}b"): int(c) for i, c in zip(idx, cnt)}
+            except ValueError:
+                if nat.norm2(psi) > 0.0:
+                    raise
+                # a zero state: the reference draws uniformly (engine.py:409-412)
         p = nat.probs(psi).cpu().numpy()
         dim = p.size
         if bool(kwargs.get("use_noise", False)):
