@@ -579,9 +579,9 @@ def run_sample(args) -> dict:
     wall = (time.perf_counter() - t0) / args.steps
     gpu_ms = s0.elapsed_time(s1) / args.steps
     assert int(cnt.sum()) == shots
-    # algorithmic bytes: two reads of the state (block sums, then the blocks
-    # that received shots: all of them at this shot count)
-    nbytes = 2 * 8 * (1 << n)
+    # algorithmic bytes: one read of the state (the segment and block sums);
+    # the shots then read only the 64-amplitude segments they land in
+    nbytes = 8 * (1 << n)
     achieved = nbytes / (gpu_ms * 1e-3) / 1e9
     return {
         "metric": METRIC,
@@ -600,7 +600,8 @@ def run_sample(args) -> dict:
                    "shots": shots, "unique_states": int(idx.size), "parallelism": "1 GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0,
                      "unit": "GB/s", "frac": round(achieved / 8000.0, 4), "traffic": None,
-                     "kernel": "sampler (block sums + per-block resolve)",
+                     "kernel": "whole sampler call (segment sums, scans, resolve, compaction, "
+                               "copy of the counts to the host)",
                      "bytes_per_launch": nbytes, "avg_launch_ms": round(gpu_ms, 4)},
         "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline_sample(n, shots),
     }

@@ -15,9 +15,10 @@ checked exactly:
     u_j = (word >> 11) * 2^-53;
   * the state is cut into blocks of min(2^n, 4096) amplitudes; x_j = u_j * P
     with P the total of |psi|^2; the block is the largest b with
-    prefix[b] <= x_j, moved down over zero-mass blocks; within the block the
-    amplitude is the smallest k with cumsum[k] > x_j - prefix[b], clamped to
-    the block and moved down over zero-width amplitudes.
+    prefix[b] <= x_j, moved down over zero-mass blocks; within the block
+    (64 segments of 64, see _within_block) the first segment and then the
+    first amplitude whose running sum exceeds x_j - prefix[b], moved down
+    over zero probabilities.
 
 How it is pinned (tests/test_sampling.py): the Philox restatement against
 numpy's Philox bit generator (an independent implementation of the same
@@ -95,14 +96,34 @@ def sample_counts(psi: np.ndarray, shots: int, seed: int) -> Tuple[np.ndarray, n
     out = np.empty(int(shots), dtype=np.int64)
     for blkid in np.unique(b):
         sel = np.nonzero(b == blkid)[0]
-        c = np.cumsum(p[blkid * blk:(blkid + 1) * blk])
-        k = np.minimum(np.searchsorted(c, xl[sel], side="right"), blk - 1)
-        for t in range(k.size):
-            while k[t] > 0 and c[k[t]] == c[k[t] - 1]:
-                k[t] -= 1
+        pv = np.zeros(BLOCK)
+        pv[:blk] = p[blkid * blk:(blkid + 1) * blk]
+        k = _within_block(pv, xl[sel])
         out[sel] = blkid * blk + k
     idx, cnt = np.unique(out, return_counts=True)
     return idx.astype(np.uint64), cnt.astype(np.int64)
+
+
+def _within_block(pv: np.ndarray, xl: np.ndarray) -> np.ndarray:
+    """Amplitude of each shot inside a block of BLOCK probabilities: the block
+    is 64 segments of 64; the segment is the first s with E[s] + S[s] > x
+    (segment sums S, exclusive prefix E; the last one if none), the amplitude
+    the first l with E[s] + cumsum(segment)[l] > x (lane 63 if none), moved
+    down over zero probabilities."""
+    seg_p = pv.reshape(64, 64)
+    S = seg_p.sum(axis=1)
+    E = np.concatenate([[0.0], np.cumsum(S)[:-1]])
+    I = E + S
+    hit = I[None, :] > xl[:, None]
+    seg = np.where(hit.any(axis=1), hit.argmax(axis=1), 63)
+    c = E[seg][:, None] + np.cumsum(seg_p[seg], axis=1)
+    hit2 = c > xl[:, None]
+    lane = np.where(hit2.any(axis=1), hit2.argmax(axis=1), 63)
+    k = 64 * seg + lane
+    for t in range(k.size):
+        while k[t] > 0 and pv[k[t]] == 0.0:
+            k[t] -= 1
+    return k
 
 
 def reference_counts(psi: np.ndarray, shots: int, seed: int) -> Tuple[np.ndarray, np.ndarray]:

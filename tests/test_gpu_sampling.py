@@ -19,7 +19,7 @@ def _state(rng, n, zero_frac=0.2):
 
 
 CASES = [(1, 10, 0), (3, 1, 5), (5, 7, 2), (8, 1000, 7), (12, 5000, 1), (13, 200000, 2),
-         (14, 3, 4), (17, 100000, 3), (20, 300000, 9)]
+         (14, 3, 4), (17, 100000, 3), (20, 300000, 9), (22, 20000, 6), (24, 1000, 3)]
 
 
 @pytest.mark.parametrize("n,shots,seed", CASES)
