@@ -388,7 +388,7 @@ int run_expval(tqsv_ham* h, const float2* psi, hipStream_t st) {
     if (pp.jit_fn)
       TQSV_HIP(jit_launch(pp, pl, st));
     else
-      TQSV_HIP(launch_pass(pp.tb, MODE_RED, n > 32, pl, st));
+      TQSV_HIP(launch_pass(pp.tb, MODE_RED, n > 32, false, pl, st));
     TQSV_HIP(k_final_sum(h->dpartials, pp.grid, pp.nred, h->dout + pp.red_base, st));
   }
   if (!h->small_terms.empty())

@@ -32,7 +32,9 @@ struct PassLaunch {
   size_t lds_bytes;
 };
 // mode: MODE_GATE / MODE_RED / MODE_DUAL (tqsv_prog.h)
-hipError_t launch_pass(int tb, int mode, bool idx64, const PassLaunch& pl, hipStream_t st);
+// lean: the pass has no OP_U2 / OP_PROT steps (smaller kernel variant)
+hipError_t launch_pass(int tb, int mode, bool idx64, bool lean, const PassLaunch& pl,
+                       hipStream_t st);
 size_t pass_lds_bytes(int tb, int mode, uint32_t nred);
 bool pass_variant_exists(int tb, int mode, bool idx64);
 

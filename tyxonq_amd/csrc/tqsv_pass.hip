@@ -8,7 +8,7 @@
 
 namespace tqsv {
 
-template <int TB, int MODE, typename IDX>
+template <int TB, int MODE, typename IDX, bool LEAN>
 hipError_t launch_one(const PassLaunch& pl, hipStream_t st);
 
 size_t pass_lds_bytes(int tb, int mode, uint32_t nred) {
@@ -27,33 +27,49 @@ bool pass_variant_exists(int tb, int mode, bool idx64) {
   }
 }
 
-hipError_t launch_pass(int tb, int mode, bool idx64, const PassLaunch& pl, hipStream_t st) {
+hipError_t launch_pass(int tb, int mode, bool idx64, bool lean, const PassLaunch& pl,
+                       hipStream_t st) {
+  if (lean && !idx64) {
+    if (mode == MODE_GATE) switch (tb) {
+        case 6: return launch_one<6, MODE_GATE, uint32_t, true>(pl, st);
+        case 7: return launch_one<7, MODE_GATE, uint32_t, true>(pl, st);
+        case 8: return launch_one<8, MODE_GATE, uint32_t, true>(pl, st);
+        case 9: return launch_one<9, MODE_GATE, uint32_t, true>(pl, st);
+        default: break;
+      }
+    if (mode == MODE_DUAL) switch (tb) {
+        case 6: return launch_one<6, MODE_DUAL, uint32_t, true>(pl, st);
+        case 7: return launch_one<7, MODE_DUAL, uint32_t, true>(pl, st);
+        case 8: return launch_one<8, MODE_DUAL, uint32_t, true>(pl, st);
+        default: break;
+      }
+  }
   if (idx64) {
-    if (mode == MODE_GATE && tb == 8) return launch_one<8, MODE_GATE, uint64_t>(pl, st);
-    if (mode == MODE_RED && tb == 8) return launch_one<8, MODE_RED, uint64_t>(pl, st);
+    if (mode == MODE_GATE && tb == 8) return launch_one<8, MODE_GATE, uint64_t, false>(pl, st);
+    if (mode == MODE_RED && tb == 8) return launch_one<8, MODE_RED, uint64_t, false>(pl, st);
     return hipErrorInvalidValue;
   }
   switch (mode) {
     case MODE_GATE:
       switch (tb) {
-        case 6: return launch_one<6, MODE_GATE, uint32_t>(pl, st);
-        case 7: return launch_one<7, MODE_GATE, uint32_t>(pl, st);
-        case 8: return launch_one<8, MODE_GATE, uint32_t>(pl, st);
-        case 9: return launch_one<9, MODE_GATE, uint32_t>(pl, st);
+        case 6: return launch_one<6, MODE_GATE, uint32_t, false>(pl, st);
+        case 7: return launch_one<7, MODE_GATE, uint32_t, false>(pl, st);
+        case 8: return launch_one<8, MODE_GATE, uint32_t, false>(pl, st);
+        case 9: return launch_one<9, MODE_GATE, uint32_t, false>(pl, st);
         default: return hipErrorInvalidValue;
       }
     case MODE_RED:
       switch (tb) {
-        case 6: return launch_one<6, MODE_RED, uint32_t>(pl, st);
-        case 7: return launch_one<7, MODE_RED, uint32_t>(pl, st);
-        case 8: return launch_one<8, MODE_RED, uint32_t>(pl, st);
+        case 6: return launch_one<6, MODE_RED, uint32_t, false>(pl, st);
+        case 7: return launch_one<7, MODE_RED, uint32_t, false>(pl, st);
+        case 8: return launch_one<8, MODE_RED, uint32_t, false>(pl, st);
         default: return hipErrorInvalidValue;
       }
     case MODE_DUAL:
       switch (tb) {
-        case 6: return launch_one<6, MODE_DUAL, uint32_t>(pl, st);
-        case 7: return launch_one<7, MODE_DUAL, uint32_t>(pl, st);
-        case 8: return launch_one<8, MODE_DUAL, uint32_t>(pl, st);
+        case 6: return launch_one<6, MODE_DUAL, uint32_t, false>(pl, st);
+        case 7: return launch_one<7, MODE_DUAL, uint32_t, false>(pl, st);
+        case 8: return launch_one<8, MODE_DUAL, uint32_t, false>(pl, st);
         default: return hipErrorInvalidValue;
       }
     default: return hipErrorInvalidValue;

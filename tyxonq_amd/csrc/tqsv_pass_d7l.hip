@@ -1,0 +1,6 @@
+// Instantiates the lean tile-pass kernel for TB=7, MODE_DUAL, IDX=uint32_t.
+#include "tqsv_pass_impl.h"
+
+namespace tqsv {
+template hipError_t launch_one<7, MODE_DUAL, uint32_t, true>(const PassLaunch& pl, hipStream_t st);
+}

@@ -2,5 +2,5 @@
 #include "tqsv_pass_impl.h"
 
 namespace tqsv {
-template hipError_t launch_one<8, MODE_DUAL, uint32_t>(const PassLaunch& pl, hipStream_t st);
+template hipError_t launch_one<8, MODE_DUAL, uint32_t, false>(const PassLaunch& pl, hipStream_t st);
 }

@@ -16,7 +16,11 @@
 
 namespace tqsv {
 
-template <int TB, int MODE, typename IDX>
+// LEAN variants leave out the dense 4x4 (OP_U2) and Pauli-rotation (OP_PROT)
+// arms, which are most of the kernel's code: without them the kernel is about
+// a fifth of the size and every branch stays within the short-branch range.
+// The planner marks passes that contain neither step as lean.
+template <int TB, int MODE, typename IDX, bool LEAN>
 __global__ __launch_bounds__(1 << TB) void k_pass(float2* __restrict__ psi, float2* __restrict__ lam,
                                                   const uint32_t* __restrict__ prog_g,
                                                   double* __restrict__ partials, uint32_t ntiles) {
@@ -110,9 +114,13 @@ __global__ __launch_bounds__(1 << TB) void k_pass(float2* __restrict__ psi, floa
             case OP_SCL: scl_step<MODE>(a, b, w); break;
             case OP_CXP: cxp_switch<MODE, IDX>(w[1], a, b, w, G); break;
             case OP_U1: u1_switch<MODE, IDX>(w[1], a, b, w, G); break;
-            case OP_U2: u2_switch<MODE, IDX>(w[1], w[2], a, b, w, G); break;
+            case OP_U2:
+              if constexpr (!LEAN) u2_switch<MODE, IDX>(w[1], w[2], a, b, w, G);
+              break;
             case OP_D2: d2_step<MODE, IDX>(a, b, w, G); break;
-            case OP_PROT: prot_switch<MODE, IDX>(w[1], a, b, w, G); break;
+            case OP_PROT:
+              if constexpr (!LEAN) prot_switch<MODE, IDX>(w[1], a, b, w, G);
+              break;
             case OP_RED:
               if constexpr (DUAL) red_switch<MODE, IDX>(w[1], a, b, w, G, acc, nred);
               break;
@@ -167,17 +175,17 @@ __global__ __launch_bounds__(1 << TB) void k_pass(float2* __restrict__ psi, floa
   }
 }
 
-template <int TB, int MODE, typename IDX>
+template <int TB, int MODE, typename IDX, bool LEAN>
 hipError_t launch_one(const PassLaunch& pl, hipStream_t st) {
   static bool attr_set = false;
   if (!attr_set) {
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_pass<TB, MODE, IDX>),
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_pass<TB, MODE, IDX, LEAN>),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     if (e != hipSuccess) return e;
     attr_set = true;
   }
-  hipLaunchKernelGGL((k_pass<TB, MODE, IDX>), dim3(pl.grid), dim3(1 << TB), pl.lds_bytes, st,
-                     pl.psi, pl.lam, pl.prog, pl.partials, pl.ntiles);
+  hipLaunchKernelGGL((k_pass<TB, MODE, IDX, LEAN>), dim3(pl.grid), dim3(1 << TB), pl.lds_bytes,
+                     st, pl.psi, pl.lam, pl.prog, pl.partials, pl.ntiles);
   return hipGetLastError();
 }
 

@@ -1177,6 +1177,16 @@ int emit_passes(std::vector<Gate>& gates, const SchedOpts& so, std::vector<PassP
     h[PH_WORDS] = (uint32_t)(prog.size() - pp.prog_off);
     for (uint32_t k = 0; k < pp.m; ++k) h[PH_H + k] = pp.H[k];
     pp.prog_words = prog.size() - pp.prog_off;
+    pp.lean = true;
+    for (size_t at = pp.prog_off + PH_SIZE, ip = 0; ip < pp.phases.size(); ++ip) {
+      const uint32_t ns = prog[at + PS_NSTEPS];
+      at += PS_SIZE;
+      for (uint32_t k = 0; k < ns; ++k) {
+        const uint32_t op = prog[at] & 0xffffu;
+        if (op == OP_U2 || op == OP_PROT) pp.lean = false;
+        at += prog[at] >> 16;
+      }
+    }
     pp.nred = nred;
     red_total += nred;
     pp.ntiles = (uint32_t)(1ull << (n - T));
@@ -1267,7 +1277,7 @@ int run_passes(const std::vector<PassPlan>& passes, const std::vector<Gate>& gat
       if (pp.jit_fn)
         TQSV_HIP(jit_launch(pp, pl, st));
       else
-        TQSV_HIP(launch_pass(pp.tb, pp.mode, n > 32, pl, st));
+        TQSV_HIP(launch_pass(pp.tb, pp.mode, n > 32, pp.lean, pl, st));
       if (pp.nred) TQSV_HIP(k_final_sum(dpartials, pp.grid, pp.nred, dred_out + pp.red_base, st));
     } else {
       const Gate& g = gates[pp.single];

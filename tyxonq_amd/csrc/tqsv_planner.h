@@ -75,6 +75,7 @@ struct PassPlan {
   uint32_t ntiles = 0, grid = 0;
   size_t lds = 0;
   void* jit_fn = nullptr;  // generated kernel (hipFunction_t), when the plan uses them
+  bool lean = false;       // no OP_U2 / OP_PROT steps: the lean precompiled variant runs it
 };
 
 // Default per-pass budget (approximate VALU ops per amplitude).  Passes are
