@@ -570,12 +570,18 @@ __device__ __forceinline__ void prot_x(float2 (&a)[NSLOT], float2 (&b)[NSLOT], c
   if constexpr (XR == 0) {
     TQSV_PR(3)
   } else {
-    switch (q) {
-      case 0: TQSV_PR(0) break;
-      case 1: TQSV_PR(1) break;
-      case 2: TQSV_PR(2) break;
-      default: TQSV_PR(3) break;
-    }
+    // i^(q+2) = -i^q: the sign goes into s, so two arms per flip mask suffice
+    const float snq = q >= 2u ? -sn : sn;
+#define TQSV_PRS(QQ)                                           \
+  {                                                           \
+    prot_apply1<XR, QQ, IDX>(a, G, z, c, snq, xzpar);          \
+    if constexpr (MODE == MODE_DUAL) prot_apply1<XR, QQ, IDX>(b, G, z, c, snq, xzpar); \
+  }
+    if (q & 1u)
+      TQSV_PRS(1)
+    else
+      TQSV_PRS(0)
+#undef TQSV_PRS
   }
 #undef TQSV_PR
 }
