@@ -179,3 +179,18 @@ def test_cx_ladder_ansatz_pass_count():
         item = ("m1", 11 - int(row["t"][0]), m, int(row["cmask"]), int(row["cval"]))
         want = dense_apply_items(12, [item], want)
     np.testing.assert_allclose(nat.emulate(12, ops, v.astype(np.complex64)), want, atol=5e-5)
+
+
+@pytest.mark.parametrize("n", [1, 2, 5, 8, 10])
+def test_small_states_run_padded(n):
+    # fewer than 11 qubits: planned as an 11-qubit state whose extra high bits
+    # stay zero (the emulator checks that they do) and copied back
+    rng = np.random.default_rng(900 + n)
+    ops = random_circuit_ops(rng, n, 120, p_two=0.4 if n > 1 else 0.0)
+    v = rand_state(rng, n)
+    want, _ = O.simulate(n, ops, mode="run", init=v, reference_dispatch=False)
+    np.testing.assert_allclose(_emulated(n, ops, v), want, atol=2e-5)
+    if n >= 2:
+        info = nat.inspect(n, lower(tq.Circuit(n, ops=list(ops)), mode="run",
+                                    reference_dispatch=False)[0].segments[0][1])
+        assert info["n"] == n and info["passes"] <= 4

@@ -232,15 +232,21 @@ int tqsv_plan_create(int n, const tqsv_op* ops, int nops, const tqsv_plan_opts* 
     *out = nullptr;
     if (n < 1 || n > 34) return fail(TQSV_EINVAL, "tqsv_plan_create: n must be in [1, 34]");
     if (nops < 0 || (nops > 0 && !ops)) return fail(TQSV_EINVAL, "tqsv_plan_create: bad op array");
-    auto p = std::make_unique<tqsv_plan>();
-    p->n = n;
+    auto p = std::unique_ptr<tqsv_plan, void (*)(tqsv_plan*)>(new tqsv_plan(), tqsv_plan_destroy);
+    p->nu = n;
     p->nops = nops;
-    p->so.n = n;
     if (opts) {
       p->so.tile_bits = opts->tile_bits > 0 ? opts->tile_bits : 13;
       p->so.min_low = opts->min_low_bits > 0 ? opts->min_low_bits : 5;
       p->so.max_cost = opts->max_cost > 0 ? opts->max_cost : kDefaultMaxCost;
       p->so.fuse = opts->fuse != 0;
+    }
+    p->n = plan_qubits(n, p->so.fuse);
+    p->so.n = p->n;
+    if (p->n > n) {
+      const size_t bytes = sizeof(float2) << p->n;
+      TQSV_HIP(hipMalloc(reinterpret_cast<void**>(&p->pad_a), bytes));
+      TQSV_HIP(hipMalloc(reinterpret_cast<void**>(&p->pad_b), bytes));
     }
     p->jit_req = opts ? opts->jit : 0;
     const int rc = plan_build_forward(p.get(), ops, nops, nullptr);
@@ -268,7 +274,7 @@ int tqsv_plan_update(tqsv_plan* p, const tqsv_op* ops, int nops, void* stream) {
 int tqsv_plan_get_info(const tqsv_plan* p, tqsv_plan_info* info) {
   if (!p || !info) return fail(TQSV_EINVAL, "tqsv_plan_get_info: NULL argument");
   std::memset(info, 0, sizeof(*info));
-  info->n = p->n;
+  info->n = p->nu;
   info->ops_in = p->nops;
   info->gates_after_merge = (int32_t)p->gates.size();
   int passes = 0, phases = 0;
@@ -291,15 +297,16 @@ int tqsv_plan_inspect(int n, const tqsv_op* ops, int nops, const tqsv_plan_opts*
     if (n < 1 || n > 34) return fail(TQSV_EINVAL, "tqsv_plan_inspect: n must be in [1, 34]");
     if (nops < 0 || (nops > 0 && !ops)) return fail(TQSV_EINVAL, "tqsv_plan_inspect: bad op array");
     tqsv_plan p;
-    p.n = n;
+    p.nu = n;
     p.nops = nops;
-    p.so.n = n;
     if (opts) {
       p.so.tile_bits = opts->tile_bits > 0 ? opts->tile_bits : 13;
       p.so.min_low = opts->min_low_bits > 0 ? opts->min_low_bits : 5;
       p.so.max_cost = opts->max_cost > 0 ? opts->max_cost : kDefaultMaxCost;
       p.so.fuse = opts->fuse != 0;
     }
+    p.n = plan_qubits(n, p.so.fuse);
+    p.so.n = p.n;
     std::vector<Gate> g;
     int rc = gates_from_ops(n, ops, nops, g);
     if (rc) return rc;
@@ -316,8 +323,15 @@ int tqsv_plan_inspect(int n, const tqsv_op* ops, int nops, const tqsv_plan_opts*
 
 int tqsv_plan_run(tqsv_plan* p, void* psi, void* stream) {
   if (!p || !psi) return fail(TQSV_EINVAL, "tqsv_plan_run: NULL argument");
-  return run_passes(p->passes, p->gates, p->n, p->dprog, F(psi), nullptr, nullptr, nullptr,
-                    S(stream));
+  hipStream_t st = S(stream);
+  if (p->n == p->nu)
+    return run_passes(p->passes, p->gates, p->n, p->dprog, F(psi), nullptr, nullptr, nullptr, st);
+  TQSV_HIP(k_pad_copy(F(psi), 1ull << p->nu, p->pad_a, 1ull << p->n, st));
+  const int rc =
+      run_passes(p->passes, p->gates, p->n, p->dprog, p->pad_a, nullptr, nullptr, nullptr, st);
+  if (rc) return rc;
+  TQSV_HIP(hipMemcpyAsync(psi, p->pad_a, sizeof(float2) << p->nu, hipMemcpyDeviceToDevice, st));
+  return 0;
 }
 
 void tqsv_plan_destroy(tqsv_plan* p) {
@@ -326,6 +340,8 @@ void tqsv_plan_destroy(tqsv_plan* p) {
   (void)hipFree(p->drprog);
   (void)hipFree(p->dpartials);
   (void)hipFree(p->dred);
+  (void)hipFree(p->pad_a);
+  (void)hipFree(p->pad_b);
   delete p;
 }
 

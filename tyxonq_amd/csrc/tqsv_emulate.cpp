@@ -7,6 +7,7 @@
 // amplitudes, so the sequential emulation computes exactly what the device
 // does up to float rounding order.  This lets the planner and the program
 // format be tested on a machine without a GPU.
+#include <algorithm>
 #include <complex>
 #include <cstring>
 #include <memory>
@@ -387,15 +388,16 @@ extern "C" int tqsv_plan_emulate(int n, const tqsv_op* ops, int nops, const tqsv
     if (!state) return fail(TQSV_EINVAL, "tqsv_plan_emulate: state is NULL");
     if (n < 1 || n > 26) return fail(TQSV_EINVAL, "tqsv_plan_emulate: n must be in [1, 26]");
     tqsv_plan p;
-    p.n = n;
+    p.nu = n;
     p.nops = nops;
-    p.so.n = n;
     if (opts) {
       p.so.tile_bits = opts->tile_bits > 0 ? opts->tile_bits : 13;
       p.so.min_low = opts->min_low_bits > 0 ? opts->min_low_bits : 5;
       p.so.max_cost = opts->max_cost > 0 ? opts->max_cost : kDefaultMaxCost;
       p.so.fuse = opts->fuse != 0;
     }
+    p.n = plan_qubits(n, p.so.fuse);
+    p.so.n = p.n;
     std::vector<Gate> g;
     int rc = gates_from_ops(n, ops, nops, g);
     if (rc) return rc;
@@ -403,13 +405,21 @@ extern "C" int tqsv_plan_emulate(int n, const tqsv_op* ops, int nops, const tqsv
     uint32_t red_total = 0;
     rc = schedule(g, p.so, p.passes, p.prog, red_total);
     if (rc) return rc;
-    cf* psi = reinterpret_cast<cf*>(state);
+    // padded like tqsv_plan_run: the caller's amplitudes low, zeros above
+    std::vector<cf> padded((size_t)1 << p.n, cf(0.f, 0.f));
+    cf* user = reinterpret_cast<cf*>(state);
+    std::copy(user, user + ((size_t)1 << n), padded.begin());
+    cf* psi = padded.data();
     for (const PassPlan& pp : p.passes) {
       if (pp.tile)
-        emulate_pass(pp, p.prog.data() + pp.prog_off, n, psi, nullptr, nullptr);
+        emulate_pass(pp, p.prog.data() + pp.prog_off, p.n, psi, nullptr, nullptr);
       else
-        apply_single_host(g[pp.single], n, psi);
+        apply_single_host(g[pp.single], p.n, psi);
     }
+    for (size_t i = ((size_t)1 << n); i < padded.size(); ++i)
+      if (padded[i] != cf(0.f, 0.f))
+        return fail(TQSV_EINVAL, "tqsv_plan_emulate: padding amplitudes changed (internal error)");
+    std::copy(padded.begin(), padded.begin() + ((size_t)1 << n), user);
     return 0;
   } catch (const std::bad_alloc&) {
     return fail(TQSV_ENOMEM, "tqsv_plan_emulate: out of host memory");

@@ -433,7 +433,7 @@ static size_t count_tile(const std::vector<PassPlan>& passes) {
 
 int plan_build_forward(tqsv_plan* p, const tqsv_op* ops, int nops, hipStream_t st) {
   std::vector<Gate> g;
-  int rc = gates_from_ops(p->n, ops, nops, g);
+  int rc = gates_from_ops(p->nu, ops, nops, g);
   if (rc) return rc;
   merge_and_classify(g);
   const bool same = same_structure(g, p->gates) && !p->fsel.empty();
@@ -686,15 +686,22 @@ int tqsv_adjoint(tqsv_plan* p, tqsv_ham* h, void* psi, void* lam, int nparams, d
   try {
     if (!p || !h || !psi || !lam || (nparams > 0 && !grad))
       return fail(TQSV_EINVAL, "tqsv_adjoint: NULL argument");
-    if (p->n != h->n) return fail(TQSV_EINVAL, "tqsv_adjoint: plan and Hamiltonian sizes differ");
+    if (p->nu != h->n) return fail(TQSV_EINVAL, "tqsv_adjoint: plan and Hamiltonian sizes differ");
     if (nparams <= p->max_param)
       return fail(TQSV_EINVAL, "tqsv_adjoint: nparams smaller than the largest parameter index + 1");
     if (psi == lam) return fail(TQSV_EINVAL, "tqsv_adjoint: lam must not alias psi");
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     int rc = plan_build_adjoint(p, st);
     if (rc) return rc;
-    float2* a = reinterpret_cast<float2*>(psi);
-    float2* b = reinterpret_cast<float2*>(lam);
+    const bool pad = p->n > p->nu;
+    float2* a = pad ? p->pad_a : reinterpret_cast<float2*>(psi);
+    float2* b = pad ? p->pad_b : reinterpret_cast<float2*>(lam);
+    if (pad) {
+      // the caller's state in the low 2^nu amplitudes, zeros above; lambda
+      // zero above as well (H|psi> below writes only the low part)
+      TQSV_HIP(k_pad_copy(reinterpret_cast<const float2*>(psi), 1ull << p->nu, a, 1ull << p->n, st));
+      TQSV_HIP(k_pad_copy(nullptr, 0, b, 1ull << p->n, st));
+    }
     rc = run_passes(p->passes, p->gates, p->n, p->dprog, a, nullptr, nullptr, nullptr, st);
     if (rc) return rc;
     TQSV_HIP(k_ham_apply(a, b, h->n, h->ngroups, h->d_gx, h->d_gstart, h->d_tz, h->d_tc, st));
@@ -704,6 +711,11 @@ int tqsv_adjoint(tqsv_plan* p, tqsv_ham* h, void* psi, void* lam, int nparams, d
     std::vector<double> hred((size_t)p->nred_adj + 2);
     rc = reverse_and_collect(p, a, b, nparams, grad, 1.0, hred.data(), st);
     if (rc) return rc;
+    if (pad) {
+      const size_t bytes = sizeof(float2) << p->nu;
+      TQSV_HIP(hipMemcpyAsync(psi, a, bytes, hipMemcpyDeviceToDevice, st));
+      TQSV_HIP(hipMemcpyAsync(lam, b, bytes, hipMemcpyDeviceToDevice, st));
+    }
     if (energy) *energy = hred[(size_t)p->nred_adj];
     return 0;
   } catch (const std::bad_alloc&) {
@@ -721,8 +733,17 @@ int tqsv_vjp(tqsv_plan* p, void* psi, void* lam, int nparams, double* grad, void
     int rc = plan_build_adjoint(p, st);
     if (rc) return rc;
     std::vector<double> hred((size_t)p->nred_adj + 2);
-    return reverse_and_collect(p, reinterpret_cast<float2*>(psi), reinterpret_cast<float2*>(lam),
-                               nparams, grad, 0.5, hred.data(), st);
+    if (p->n == p->nu)
+      return reverse_and_collect(p, reinterpret_cast<float2*>(psi), reinterpret_cast<float2*>(lam),
+                                 nparams, grad, 0.5, hred.data(), st);
+    const uint64_t du = 1ull << p->nu, dn = 1ull << p->n;
+    TQSV_HIP(k_pad_copy(reinterpret_cast<const float2*>(psi), du, p->pad_a, dn, st));
+    TQSV_HIP(k_pad_copy(reinterpret_cast<const float2*>(lam), du, p->pad_b, dn, st));
+    rc = reverse_and_collect(p, p->pad_a, p->pad_b, nparams, grad, 0.5, hred.data(), st);
+    if (rc) return rc;
+    TQSV_HIP(hipMemcpyAsync(psi, p->pad_a, du * sizeof(float2), hipMemcpyDeviceToDevice, st));
+    TQSV_HIP(hipMemcpyAsync(lam, p->pad_b, du * sizeof(float2), hipMemcpyDeviceToDevice, st));
+    return 0;
   } catch (const std::bad_alloc&) {
     return fail(TQSV_ENOMEM, "tqsv_vjp: out of host memory");
   }

@@ -266,7 +266,20 @@ __global__ void ham_apply_kernel(const float2* psi, float2* out, uint64_t dim, u
   }
 }
 
+__global__ void pad_copy_kernel(const float2* __restrict__ src, uint64_t nsrc,
+                                float2* __restrict__ dst, uint64_t ndst) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < ndst) dst[i] = i < nsrc ? src[i] : make_float2(0.f, 0.f);
+}
+
 }  // namespace
+
+hipError_t k_pad_copy(const float2* src, uint64_t nsrc, float2* dst, uint64_t ndst,
+                      hipStream_t st) {
+  hipLaunchKernelGGL(pad_copy_kernel, dim3((uint32_t)((ndst + BLK - 1) / BLK)), dim3(BLK), 0, st,
+                     src, nsrc, dst, ndst);
+  return hipGetLastError();
+}
 
 hipError_t k_init_basis(float2* psi, int n, uint64_t index, hipStream_t st) {
   const uint64_t dim = 1ull << n;

@@ -84,6 +84,10 @@ struct PassPlan {
 // less per-pass overhead) and larger ones change nothing.
 constexpr int kDefaultMaxCost = 400;
 
+// Smallest state that runs as tile passes (2^11 amplitudes: 64 threads x 32).
+constexpr int kPadQubits = 11;
+inline int plan_qubits(int n, bool fuse) { return fuse && n < kPadQubits ? kPadQubits : n; }
+
 struct SchedOpts {
   int n = 0;
   int tile_bits = 13;
@@ -134,7 +138,8 @@ std::string jit_source(int tb, const uint32_t* pass_words, int n);
 }  // namespace tqsv
 
 struct tqsv_plan {
-  int n = 0;
+  int n = 0;   // qubits the plan is built for (at least kPadQubits when fusing)
+  int nu = 0;  // qubits of the caller's states (n > nu: states are padded, see pad_a)
   int nops = 0;
   int jit_req = 0;  // requested: 1 generated kernels, -1 precompiled, 0 automatic
   bool fgen = false, rgen = false;  // the last build ran generated kernels (forward, reverse)
@@ -159,6 +164,12 @@ struct tqsv_plan {
   double* dred = nullptr;
   size_t dred_cap = 0;
   int max_param = -1;
+  // States of fewer than kPadQubits qubits run as kPadQubits-qubit states whose
+  // extra high index bits stay zero (no gate touches them), so that they use
+  // the tile passes instead of one launch per gate: the caller's 2^nu
+  // amplitudes are copied into pad_a (pad_b for lambda), the rest is zero.
+  float2* pad_a = nullptr;
+  float2* pad_b = nullptr;
 };
 
 #endif
