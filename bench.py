@@ -517,7 +517,9 @@ def run_serve(args) -> dict:
     table = []
     for k, rate in enumerate(rates):
         reqs = S.make_requests(args.requests, rate, seed=args.seed + k)
-        srv = S.summarize(S.Server(slots=args.slots).run(reqs))
+        server = S.Server(slots=args.slots)
+        srv = S.summarize(server.run(reqs))
+        srv.update(server.last_stats)
         plain = S.summarize(S.plain_loop(reqs))
         table.append({"rate_rps": rate, "server": srv, "plain": plain})
         print(json.dumps({"rate": rate, "server": srv, "plain": plain}), file=sys.stderr, flush=True)
@@ -540,7 +542,9 @@ def run_serve(args) -> dict:
         "config": {"workload": f"serving at {top['rate_rps']:.0f} req/s offered "
                                f"({args.requests} requests per rate), adjoint energy+grad",
                    "slots": args.slots, "admission": "2^n x 24 B footprint",
-                   "parallelism": "1 GPU, one HIP stream per slot"},
+                   "batched": "requests of <= 13 qubits: one launch per batch, one "
+                              "work-group per request",
+                   "parallelism": "1 GPU, one HIP stream per slot and one for the batch lane"},
         "p50_ms": top["server"]["p50_ms"],
         "p99_ms": top["server"]["p99_ms"],
         "plain": top["plain"],

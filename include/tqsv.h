@@ -145,8 +145,11 @@ typedef struct tqsv_plan_opts {
   int32_t fuse;          /* 1 = fused tile passes; 0 = one kernel launch per op */
   int32_t jit;           /* 1 = passes run as kernels generated and compiled for this
                             plan (hiprtc, cached per structure); -1 = the precompiled
-                            interpreting kernel; 0 = automatic (generated for n >= 20).
-                            The environment variable TQSV_JIT=0/1 overrides. */
+                            interpreting kernel; 0 = automatic (generated for n >= 22 and
+                            at most 64 passes).  TQSV_JIT=0/1 overrides. */
+  int32_t min_qubits;    /* states of fewer qubits run as states of this many whose extra
+                            high index bits stay zero (copied through a buffer of the
+                            plan); 0 = 11; at most 13 (13 for tqsv_batch_adjoint) */
 } tqsv_plan_opts;
 
 typedef struct tqsv_plan_info {
@@ -241,6 +244,25 @@ void tqsv_ham_destroy(tqsv_ham* ham);
 int tqsv_expval(const void* psi, int n, int nterms, const uint64_t* xmask,
                 const uint64_t* zmask, const double* coeff, double* terms, double* energy,
                 double* norm2, void* stream);
+
+/* ---- batched energy and gradient of small requests ----------------------- */
+
+/* Energy and gradient of `nreq` requests in ONE kernel launch: one work-group
+ * per request, whose state and lambda stay in LDS for the whole forward sweep,
+ * H|psi>, energy and reverse sweep.  Request i: plans[i] built with
+ * min_qubits = 13 for a state of at most 13 qubits, hams[i] of the caller's
+ * size, psi[i] its initial state (device, read only), grads[i] (host,
+ * nparams[i] doubles) and energies[i] receive the results.  Requests of
+ * different sizes and depths share the launch; each result depends only on
+ * its own request.  No counterpart in the reference, whose serving of
+ * energy+gradient requests is one engine call per request
+ * (numerics/backends/pytorch_backend.py:446-507 per call). */
+typedef struct tqsv_batch tqsv_batch;
+int tqsv_batch_create(tqsv_batch** out);
+void tqsv_batch_destroy(tqsv_batch* batch);
+int tqsv_batch_adjoint(tqsv_batch* batch, int nreq, tqsv_plan* const* plans,
+                       tqsv_ham* const* hams, const void* const* psi, const int* nparams,
+                       double* const* grads, double* energies, void* stream);
 
 /* ---- shot sampling ------------------------------------------------------- */
 

@@ -65,3 +65,90 @@ def test_request_energy_matches_oracle(gpu):
         d[j] = np.pi / 2
         ps = 0.5 * (f(req.params + d) - f(req.params - d))
         assert abs(res.grad[j] - ps) < 1e-5
+
+
+def _oracle_energy_grad(req, idx):
+    ops = []
+    k = 0
+    for _ in range(req.depth):
+        for q in range(req.n):
+            ops.append(("ry", q, float(req.params[k])))
+            k += 1
+        for q in range(req.n):
+            ops.append(("rz", q, float(req.params[k])))
+            k += 1
+        for q in range(req.n - 1):
+            ops.append(("cx", q, q + 1))
+
+    def f(p):
+        o, j = [], 0
+        for op in ops:
+            if op[0] in ("ry", "rz"):
+                o.append((op[0], op[1], float(p[j])))
+                j += 1
+            else:
+                o.append(op)
+        s, _ = O.simulate(req.n, o)
+        return O.pauli_sum_expectation(s, S.request_terms(req), req.n)
+
+    e = f(req.params)
+    g = {}
+    for j in idx:
+        d = np.zeros(len(req.params))
+        d[j] = np.pi / 2
+        g[j] = 0.5 * (f(req.params + d) - f(req.params - d))
+    return e, g
+
+
+def _batch_inputs(reqs):
+    from tyxonq_amd import _native as nat
+    from tyxonq_amd.pauli import PauliSum
+
+    plans, hams, psis = [], [], []
+    for r in reqs:
+        plans.append(nat.Plan(r.n, S.request_ops(r), min_qubits=13, jit=-1))
+        x, z, c = PauliSum(r.n, S.request_terms(r)).to_masks()
+        hams.append(nat.Ham(r.n, x, z, c))
+        psis.append(nat.new_state(r.n, 0))
+    return plans, hams, psis
+
+
+def test_batched_launch_matches_oracle_and_ignores_the_mix(gpu):
+    # requests of 2..13 qubits and depths 1..5 in one launch: each energy and
+    # gradient matches the complex128 oracle, and a request's result is the
+    # same bit for bit in a batch of any composition
+    from tyxonq_amd import _native as nat
+
+    reqs = S.make_requests(12, 1e6, seed=9, nmin=2, nmax=13, dmin=1, dmax=5)
+    plans, hams, psis = _batch_inputs(reqs)
+    batch = nat.Batch()
+    full = batch.adjoint(plans, hams, psis, [len(r.params) for r in reqs])
+    for r, (e, g) in zip(reqs, full):
+        idx = sorted({0, len(r.params) // 2, len(r.params) - 1})
+        we, wg = _oracle_energy_grad(r, idx)
+        assert abs(e - we) < 1e-6, (r.n, r.depth)
+        for j in idx:
+            assert abs(g[j] - wg[j]) < 1e-5, (r.n, r.depth, j)
+    sub = [3, 7, 0]
+    part = batch.adjoint([plans[i] for i in sub], [hams[i] for i in sub], [psis[i] for i in sub],
+                         [len(reqs[i].params) for i in sub])
+    for k, i in enumerate(sub):
+        assert part[k][0] == full[i][0]
+        np.testing.assert_array_equal(part[k][1], full[i][1])
+    # the single-request path (separate launches) agrees to rounding
+    for r, p, h, (e, g) in zip(reqs, plans, hams, full):
+        psi = nat.new_state(r.n, 0)
+        lam = nat.new_state(r.n, 0)
+        e1, g1 = nat.Plan(r.n, S.request_ops(r), jit=-1).adjoint(h, psi, lam, len(r.params))
+        assert abs(e1 - e) < 1e-6
+        np.testing.assert_allclose(g1, g, atol=1e-5)
+
+
+def test_batch_rejects_plans_not_built_for_it(gpu):
+    from tyxonq_amd import _native as nat
+
+    reqs = S.make_requests(2, 1e6, seed=2, nmin=5, nmax=6, dmin=1, dmax=2)
+    plans, hams, psis = _batch_inputs(reqs)
+    plans[1] = nat.Plan(reqs[1].n, S.request_ops(reqs[1]))  # built for 11 qubits
+    with pytest.raises(ValueError):
+        nat.Batch().adjoint(plans, hams, psis, [len(r.params) for r in reqs])

@@ -49,6 +49,7 @@ class PlanOpts(ctypes.Structure):
         ("max_cost", ctypes.c_int32),
         ("fuse", ctypes.c_int32),
         ("jit", ctypes.c_int32),
+        ("min_qubits", ctypes.c_int32),
     ]
 
 
@@ -119,6 +120,9 @@ _SIGS = {
     "tqsv_expval": (_I, [_P, _I, _I, _P, _P, _P, _DP, _DP, _DP, _P]),
     "tqsv_adjoint": (_I, [_P, _P, _P, _P, _I, _DP, _DP, _P]),
     "tqsv_vjp": (_I, [_P, _P, _P, _I, _DP, _P]),
+    "tqsv_batch_create": (_I, [ctypes.POINTER(_P)]),
+    "tqsv_batch_destroy": (None, [_P]),
+    "tqsv_batch_adjoint": (_I, [_P, _I, _P, _P, _P, _P, _P, _DP, _P]),
     "tqsv_sampler_create": (_I, [_I, ctypes.c_int64, ctypes.POINTER(_P)]),
     "tqsv_sampler_run": (_I, [_P, _P, ctypes.c_int64, _U64, _P, _P,
                               ctypes.POINTER(ctypes.c_int64), _P]),
@@ -312,8 +316,9 @@ def probs(psi):
 # ---- plans -----------------------------------------------------------------
 
 def _opts(tile_bits: int = 0, min_low_bits: int = 0, max_cost: int = 0, fuse: bool = True,
-          jit: int = 0):
-    return PlanOpts(int(tile_bits), int(min_low_bits), int(max_cost), 1 if fuse else 0, int(jit))
+          jit: int = 0, min_qubits: int = 0):
+    return PlanOpts(int(tile_bits), int(min_low_bits), int(max_cost), 1 if fuse else 0, int(jit),
+                    int(min_qubits))
 
 
 def inspect(n: int, ops: np.ndarray, *, tile_bits: int = 0, min_low_bits: int = 0,
@@ -342,12 +347,12 @@ class Plan:
     """A planned (fused) circuit: owns its device program."""
 
     def __init__(self, n: int, ops: np.ndarray, *, tile_bits: int = 0, min_low_bits: int = 0,
-                 max_cost: int = 0, fuse: bool = True, jit: int = 0):
+                 max_cost: int = 0, fuse: bool = True, jit: int = 0, min_qubits: int = 0):
         load()
         self.n = int(n)
         self._ops = np.ascontiguousarray(ops, dtype=OP_DTYPE)
         self._h = ctypes.c_void_p()
-        o = _opts(tile_bits, min_low_bits, max_cost, fuse, jit)
+        o = _opts(tile_bits, min_low_bits, max_cost, fuse, jit, min_qubits)
         _check(
             load().tqsv_plan_create(
                 self.n, self._ops.ctypes.data if len(self._ops) else None, len(self._ops),
@@ -538,4 +543,45 @@ def sample_counts(psi, shots: int, seed: int = 0):
         s = Sampler(n, max(int(shots), 1024))
         _samplers[n] = s
     return s.sample(psi, shots, seed)
+
+
+class Batch:
+    """Energy and gradient of several small requests (at most 13 qubits, plans
+    built with min_qubits=13) in one kernel launch."""
+
+    def __init__(self):
+        load()
+        self._h = ctypes.c_void_p()
+        _check(load().tqsv_batch_create(ctypes.byref(self._h)))
+
+    def adjoint(self, plans: Sequence["Plan"], hams: Sequence["Ham"], psis, nparams: Sequence[int]):
+        """[(energy, grad)] per request; psis are the initial states (device)."""
+        k = len(plans)
+        if not (len(hams) == len(psis) == len(nparams) == k):
+            raise ValueError("plans, hams, psis and nparams must have equal length")
+        if k == 0:
+            return []
+        P = (ctypes.c_void_p * k)(*[p._h.value for p in plans])
+        H = (ctypes.c_void_p * k)(*[h._h.value for h in hams])
+        S = (ctypes.c_void_p * k)(*[_ptr(x) for x in psis])
+        npar = [int(x) for x in nparams]
+        N = (ctypes.c_int * k)(*npar)
+        grads = [np.zeros(max(m, 1)) for m in npar]
+        G = (ctypes.c_void_p * k)(*[g.ctypes.data for g in grads])
+        E = np.zeros(k)
+        _check(load().tqsv_batch_adjoint(self._h, k, ctypes.cast(P, _P), ctypes.cast(H, _P),
+                                         ctypes.cast(S, _P), ctypes.cast(N, _P),
+                                         ctypes.cast(G, _P), E.ctypes.data_as(_DP), stream_ptr()))
+        return [(float(E[i]), grads[i][:npar[i]]) for i in range(k)]
+
+    def close(self) -> None:
+        if self._h:
+            load().tqsv_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 

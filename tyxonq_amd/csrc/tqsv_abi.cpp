@@ -232,6 +232,8 @@ int tqsv_plan_create(int n, const tqsv_op* ops, int nops, const tqsv_plan_opts* 
     *out = nullptr;
     if (n < 1 || n > 34) return fail(TQSV_EINVAL, "tqsv_plan_create: n must be in [1, 34]");
     if (nops < 0 || (nops > 0 && !ops)) return fail(TQSV_EINVAL, "tqsv_plan_create: bad op array");
+    if (opts && (opts->min_qubits < 0 || opts->min_qubits > 13))
+      return fail(TQSV_EINVAL, "tqsv_plan_create: min_qubits must be in [0, 13]");
     auto p = std::unique_ptr<tqsv_plan, void (*)(tqsv_plan*)>(new tqsv_plan(), tqsv_plan_destroy);
     p->nu = n;
     p->nops = nops;
@@ -241,7 +243,7 @@ int tqsv_plan_create(int n, const tqsv_op* ops, int nops, const tqsv_plan_opts* 
       p->so.max_cost = opts->max_cost > 0 ? opts->max_cost : kDefaultMaxCost;
       p->so.fuse = opts->fuse != 0;
     }
-    p->n = plan_qubits(n, p->so.fuse);
+    p->n = plan_qubits(n, p->so.fuse, opts ? opts->min_qubits : 0);
     p->so.n = p->n;
     if (p->n > n) {
       const size_t bytes = sizeof(float2) << p->n;
@@ -305,7 +307,7 @@ int tqsv_plan_inspect(int n, const tqsv_op* ops, int nops, const tqsv_plan_opts*
       p.so.max_cost = opts->max_cost > 0 ? opts->max_cost : kDefaultMaxCost;
       p.so.fuse = opts->fuse != 0;
     }
-    p.n = plan_qubits(n, p.so.fuse);
+    p.n = plan_qubits(n, p.so.fuse, opts ? opts->min_qubits : 0);
     p.so.n = p.n;
     std::vector<Gate> g;
     int rc = gates_from_ops(n, ops, nops, g);

@@ -396,7 +396,7 @@ extern "C" int tqsv_plan_emulate(int n, const tqsv_op* ops, int nops, const tqsv
       p.so.max_cost = opts->max_cost > 0 ? opts->max_cost : kDefaultMaxCost;
       p.so.fuse = opts->fuse != 0;
     }
-    p.n = plan_qubits(n, p.so.fuse);
+    p.n = plan_qubits(n, p.so.fuse, opts ? opts->min_qubits : 0);
     p.so.n = p.n;
     std::vector<Gate> g;
     int rc = gates_from_ops(n, ops, nops, g);

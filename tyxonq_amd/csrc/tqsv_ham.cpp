@@ -11,6 +11,7 @@
 // LDS, per-work-group partials and a fixed-order final sum: the result does
 // not depend on launch timing.
 #include <algorithm>
+#include <new>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -535,6 +536,25 @@ int plan_build_adjoint(tqsv_plan* p, hipStream_t st) {
   return 0;
 }
 
+// dE/dparam from the reverse sweep's reduction slots:
+// grad += scale * gcoef * red_scale * Im((-i)^k S) per parametrised gate
+static void collect_grad(const tqsv_plan* p, const double* hred, int nparams, double* grad,
+                         double scale) {
+  for (int i = 0; i < nparams; ++i) grad[i] = 0.0;
+  for (const Gate& g : p->rgates) {
+    if (g.kind != GK_RED) continue;
+    const double re = hred[g.red_slot], im = hred[g.red_slot + 1];
+    double v;
+    switch (g.k) {
+      case 0: v = im; break;
+      case 1: v = -re; break;
+      case 2: v = -im; break;
+      default: v = re; break;
+    }
+    grad[g.param] += scale * g.gcoef * g.red_scale * v;
+  }
+}
+
 // reverse sweep + gradient assembly; scale = 1 for energies, 1/2 for VJPs
 static int reverse_and_collect(tqsv_plan* p, float2* psi, float2* lam, int nparams, double* grad,
                                double scale, double* hred, hipStream_t st) {
@@ -543,22 +563,38 @@ static int reverse_and_collect(tqsv_plan* p, float2* psi, float2* lam, int npara
   const size_t nr = (size_t)p->nred_adj + 2;
   TQSV_HIP(hipMemcpyAsync(hred, p->dred, nr * sizeof(double), hipMemcpyDeviceToHost, st));
   TQSV_HIP(hipStreamSynchronize(st));
-  for (int i = 0; i < nparams; ++i) grad[i] = 0.0;
-  for (const Gate& g : p->rgates) {
-    if (g.kind != GK_RED) continue;
-    const double re = hred[g.red_slot], im = hred[g.red_slot + 1];
-    double v;
-    switch (g.k) {  // Im((-i)^k S)
-      case 0: v = im; break;
-      case 1: v = -re; break;
-      case 2: v = -im; break;
-      default: v = re; break;
-    }
-    grad[g.param] += scale * g.gcoef * g.red_scale * v;
-  }
+  collect_grad(p, hred, nparams, grad, scale);
   return 0;
 }
 
+}  // namespace tqsv
+
+struct tqsv_batch {
+  uint32_t* doff = nullptr;
+  size_t doff_cap = 0;
+  tqsv::BatchItem* ditems = nullptr;
+  size_t items_cap = 0;
+  double* dout = nullptr;
+  size_t out_cap = 0;
+  ~tqsv_batch() {
+    (void)hipFree(doff);
+    (void)hipFree(ditems);
+    (void)hipFree(dout);
+  }
+};
+
+namespace tqsv {
+template <typename T>
+static int grow(T** d, size_t* cap, size_t need) {
+  need = std::max<size_t>(need, 1);
+  if (*cap >= need) return 0;
+  if (*d) TQSV_HIP(hipFree(*d));
+  *d = nullptr;
+  *cap = 0;
+  TQSV_HIP(hipMalloc(reinterpret_cast<void**>(d), need * sizeof(T)));
+  *cap = need;
+  return 0;
+}
 }  // namespace tqsv
 
 // ---- C ABI -----------------------------------------------------------------
@@ -720,6 +756,103 @@ int tqsv_adjoint(tqsv_plan* p, tqsv_ham* h, void* psi, void* lam, int nparams, d
     return 0;
   } catch (const std::bad_alloc&) {
     return fail(TQSV_ENOMEM, "tqsv_adjoint: out of host memory");
+  }
+}
+
+int tqsv_batch_create(tqsv_batch** out) {
+  if (!out) return fail(TQSV_EINVAL, "tqsv_batch_create: NULL output");
+  *out = new (std::nothrow) tqsv_batch();
+  return *out ? 0 : fail(TQSV_ENOMEM, "tqsv_batch_create: out of host memory");
+}
+
+void tqsv_batch_destroy(tqsv_batch* b) { delete b; }
+
+int tqsv_batch_adjoint(tqsv_batch* b, int nreq, tqsv_plan* const* plans, tqsv_ham* const* hams,
+                       const void* const* psi, const int* nparams, double* const* grads,
+                       double* energies, void* stream) {
+  try {
+    if (!b || nreq < 0 || (nreq > 0 && (!plans || !hams || !psi || !nparams || !grads || !energies)))
+      return fail(TQSV_EINVAL, "tqsv_batch_adjoint: NULL argument");
+    if (nreq == 0) return 0;
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    bool lean = true;
+    uint32_t max_nred = 0;
+    size_t noff = 0, nout = 0;
+    for (int i = 0; i < nreq; ++i) {
+      tqsv_plan* p = plans[i];
+      const tqsv_ham* h = hams[i];
+      if (!p || !h || !psi[i] || (nparams[i] > 0 && !grads[i]))
+        return fail(TQSV_EINVAL, "tqsv_batch_adjoint: NULL request field");
+      if (p->n != BATCH_QUBITS || p->nu != h->n)
+        return fail(TQSV_EINVAL, "tqsv_batch_adjoint: plans must be built for 13 qubits "
+                                 "(min_qubits = 13) with a Hamiltonian of the caller's size");
+      if (nparams[i] <= p->max_param)
+        return fail(TQSV_EINVAL, "tqsv_batch_adjoint: nparams smaller than the largest parameter "
+                                 "index + 1");
+      int rc = plan_build_adjoint(p, st);
+      if (rc) return rc;
+      for (const auto* v : {&p->passes, &p->rpasses})
+        for (const PassPlan& pp : *v) {
+          if (!pp.tile || pp.tb != BATCH_QUBITS - RB)
+            return fail(TQSV_EUNSUPPORTED, "tqsv_batch_adjoint: a plan has a pass that is not "
+                                           "a whole-state tile pass");
+          lean = lean && pp.lean;
+          max_nred = std::max(max_nred, pp.nred);
+        }
+      noff += p->passes.size() + 2 * p->rpasses.size();
+      nout += (size_t)p->nred_adj + 2;
+    }
+    int rc = grow(&b->doff, &b->doff_cap, noff);
+    if (!rc) rc = grow(&b->ditems, &b->items_cap, (size_t)nreq);
+    if (!rc) rc = grow(&b->dout, &b->out_cap, nout);
+    if (rc) return rc;
+    std::vector<uint32_t> off;
+    off.reserve(noff);
+    std::vector<BatchItem> items((size_t)nreq);
+    std::vector<size_t> outpos((size_t)nreq);
+    size_t opos = 0;
+    for (int i = 0; i < nreq; ++i) {
+      const tqsv_plan* p = plans[i];
+      const tqsv_ham* h = hams[i];
+      BatchItem& it = items[(size_t)i];
+      it.psi = reinterpret_cast<const float2*>(psi[i]);
+      it.nu = (uint32_t)p->nu;
+      it.nfwd = (uint32_t)p->passes.size();
+      it.nrev = (uint32_t)p->rpasses.size();
+      it.fprog = p->dprog;
+      it.rprog = p->drprog;
+      it.foff = b->doff + off.size();
+      for (const PassPlan& pp : p->passes) off.push_back((uint32_t)pp.prog_off);
+      it.roff = b->doff + off.size();
+      for (const PassPlan& pp : p->rpasses) off.push_back((uint32_t)pp.prog_off);
+      it.rred = b->doff + off.size();
+      for (const PassPlan& pp : p->rpasses) off.push_back(pp.red_base);
+      it.ngroups = h->ngroups;
+      it.gx = h->d_gx;
+      it.gstart = h->d_gstart;
+      it.tz = h->d_tz;
+      it.tc = h->d_tc;
+      it.out = b->dout + opos;
+      it.nred = (uint32_t)p->nred_adj;
+      outpos[(size_t)i] = opos;
+      opos += (size_t)p->nred_adj + 2;
+    }
+    TQSV_HIP(hipMemcpyAsync(b->doff, off.data(), off.size() * sizeof(uint32_t),
+                            hipMemcpyHostToDevice, st));
+    TQSV_HIP(hipMemcpyAsync(b->ditems, items.data(), items.size() * sizeof(BatchItem),
+                            hipMemcpyHostToDevice, st));
+    TQSV_HIP(k_batch_adjoint(b->ditems, (uint32_t)nreq, lean, max_nred, st));
+    std::vector<double> hout(nout);
+    TQSV_HIP(hipMemcpyAsync(hout.data(), b->dout, nout * sizeof(double), hipMemcpyDeviceToHost, st));
+    TQSV_HIP(hipStreamSynchronize(st));
+    for (int i = 0; i < nreq; ++i) {
+      const double* hred = hout.data() + outpos[(size_t)i];
+      collect_grad(plans[i], hred, nparams[i], grads[i], 1.0);
+      energies[i] = hred[plans[i]->nred_adj];
+    }
+    return 0;
+  } catch (const std::bad_alloc&) {
+    return fail(TQSV_ENOMEM, "tqsv_batch_adjoint: out of host memory");
   }
 }
 

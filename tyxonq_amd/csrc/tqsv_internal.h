@@ -75,6 +75,30 @@ hipError_t k_ham_apply(const float2* psi, float2* out, int n, uint32_t ngroups,
                        const uint64_t* gx, const uint32_t* gstart, const uint64_t* tz,
                        const float2* tc, hipStream_t st);
 
+// ---- batched energy+gradient of small requests (tqsv_batch.hip) ----
+// One work-group per request; every plan is built for BATCH_QUBITS qubits
+// (whole state = one tile, identity tile map) and contains tile passes only.
+constexpr int BATCH_QUBITS = 13;
+struct BatchItem {
+  const float2* psi;       // caller's initial state, 2^nu amplitudes (read only)
+  uint32_t nu;             // caller's qubits (<= BATCH_QUBITS)
+  uint32_t nfwd, nrev;     // pass counts
+  const uint32_t* fprog;   // forward program, pass records at fprog + foff[k]
+  const uint32_t* rprog;   // reverse program
+  const uint32_t* foff;    // device arrays: pass offsets (words) and, for the
+  const uint32_t* roff;    // reverse passes, their first reduction slot
+  const uint32_t* rred;
+  uint32_t ngroups;        // H as flip groups (as k_ham_apply)
+  const uint64_t* gx;
+  const uint32_t* gstart;
+  const uint64_t* tz;
+  const float2* tc;
+  double* out;             // reduction slots [0, nred), energy (re, im) at [nred, nred + 1]
+  uint32_t nred;
+};
+hipError_t k_batch_adjoint(const BatchItem* items, uint32_t nreq, bool lean, uint32_t max_pass_nred,
+                           hipStream_t st);
+
 }  // namespace tqsv
 
 #endif

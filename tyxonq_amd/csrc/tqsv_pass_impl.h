@@ -16,6 +16,97 @@
 
 namespace tqsv {
 
+// The phases of one pass over the tile held in LDS (tA, and tB for lambda in
+// dual passes): per phase, every thread gathers its 32 slots, runs the
+// phase's steps in registers and writes the slots back.  `base` is the global
+// index of the tile's position 0.
+template <int TB, int MODE, typename IDX, bool LEAN>
+__device__ __forceinline__ void run_phases(cw_t prog, float2* tA, float2* tB, double* acc,
+                                           uint32_t nred, IDX base, uint32_t L, uint32_t m,
+                                           cw_t H) {
+  constexpr bool DUAL = MODE == MODE_DUAL;
+  constexpr bool WRITE = MODE != MODE_RED;
+  const uint32_t tid = threadIdx.x;
+  const uint32_t nphase = prog[PH_NPHASE];
+  cw_t ph = prog + PH_SIZE;
+  for (uint32_t iph = 0; iph < nphase; ++iph) {
+    const uint32_t nsteps = ph[PS_NSTEPS];
+    PhaseGeo<IDX> G;
+    uint32_t tpos = tid;
+    G.rmask = 0;
+#pragma unroll
+    for (int j = 0; j < RB; ++j) {
+      const uint32_t r = ph[PS_R + j];
+      tpos = ((tpos >> r) << (r + 1)) | (tpos & ((1u << r) - 1u));
+      G.sph[j] = swz(1u << r);
+      G.gR[j] = pos_map<IDX>(1u << r, L, m, H);
+      G.rmask |= G.gR[j];
+    }
+    G.bt = swz(tpos);
+    G.gb = base | pos_map<IDX>(tpos, L, m, H);
+
+    float2 a[NSLOT], b[NSLOT];
+#pragma unroll
+    for (int s = 0; s < NSLOT; ++s) {
+      const uint32_t l = G.bt ^ ls_of(G, s);
+      a[s] = tA[l];
+      if constexpr (DUAL) b[s] = tB[l];
+    }
+
+    cw_t w = ph + PS_SIZE;
+    for (uint32_t is = 0; is < nsteps; ++is) {
+      const uint32_t op = w[0] & 0xffffu;
+      const uint32_t nw = w[0] >> 16;
+      if constexpr (MODE == MODE_RED) {
+        // the slot registers do not change in a read-only pass; marking
+        // them as rewritten here keeps the compiler from hoisting the pair
+        // products of all 32 reduction arms out of the step loop
+#pragma unroll
+        for (int s = 0; s < NSLOT; ++s) __asm__ volatile("" : "+v"(a[s].x), "+v"(a[s].y));
+        if (op == OP_RED) red_switch<MODE, IDX>(w[1], a, b, w, G, acc, nred);
+      } else {
+        switch (op) {
+          case OP_ROT: rot_step<MODE, IDX>(a, b, w, G); break;
+          case OP_PH: ph_step<MODE, IDX>(a, b, w, G); break;
+          case OP_SGN: sgn_step<MODE>(a, b, w); break;
+          case OP_TAB: tab_step<MODE>(a, b, w); break;
+          case OP_THR: thr_step<MODE, IDX>(a, b, w, G); break;
+          case OP_SCL: scl_step<MODE>(a, b, w); break;
+          case OP_CXP: cxp_switch<MODE, IDX>(w[1], a, b, w, G); break;
+          case OP_U1: u1_switch<MODE, IDX>(w[1], a, b, w, G); break;
+          case OP_U2:
+            if constexpr (!LEAN) u2_switch<MODE, IDX>(w[1], w[2], a, b, w, G);
+            break;
+          case OP_D2: d2_step<MODE, IDX>(a, b, w, G); break;
+          case OP_PROT:
+            if constexpr (!LEAN) prot_switch<MODE, IDX>(w[1], a, b, w, G);
+            break;
+          case OP_RED:
+            if constexpr (DUAL) red_switch<MODE, IDX>(w[1], a, b, w, G, acc, nred);
+            break;
+          default: break;
+        }
+      }
+      w += nw;
+    }
+
+    if constexpr (WRITE) {
+      // recompute the slot addresses instead of keeping 32 of them live
+      // through the phase
+      uint32_t btw = G.bt;
+      __asm__ volatile("" : "+v"(btw));
+#pragma unroll
+      for (int s = 0; s < NSLOT; ++s) {
+        const uint32_t l = btw ^ ls_of(G, s);
+        tA[l] = a[s];
+        if constexpr (DUAL) tB[l] = b[s];
+      }
+    }
+    __syncthreads();
+    ph = w;
+  }
+}
+
 // LEAN variants leave out the dense 4x4 (OP_U2) and Pauli-rotation (OP_PROT)
 // arms, which are most of the kernel's code: without them the kernel is about
 // a fifth of the size and every branch stays within the short-branch range.
@@ -39,7 +130,6 @@ __global__ __launch_bounds__(1 << TB) void k_pass(float2* __restrict__ psi, floa
   const cw_t prog = (cw_t)prog_g;
   const uint32_t tid = threadIdx.x;
   const uint32_t L = prog[PH_L], m = prog[PH_M];
-  const uint32_t nphase = prog[PH_NPHASE];
   const uint32_t nred = REDS ? prog[PH_NRED] : 0u;
   const cw_t H = prog + PH_H;
   constexpr uint32_t NWAVE = NT / 64 > 0 ? NT / 64 : 1;
@@ -68,83 +158,7 @@ __global__ __launch_bounds__(1 << TB) void k_pass(float2* __restrict__ psi, floa
     }
     __syncthreads();
 
-    cw_t ph = prog + PH_SIZE;
-    for (uint32_t iph = 0; iph < nphase; ++iph) {
-      const uint32_t nsteps = ph[PS_NSTEPS];
-      PhaseGeo<IDX> G;
-      uint32_t tpos = tid;
-      G.rmask = 0;
-#pragma unroll
-      for (int j = 0; j < RB; ++j) {
-        const uint32_t r = ph[PS_R + j];
-        tpos = ((tpos >> r) << (r + 1)) | (tpos & ((1u << r) - 1u));
-        G.sph[j] = swz(1u << r);
-        G.gR[j] = pos_map<IDX>(1u << r, L, m, H);
-        G.rmask |= G.gR[j];
-      }
-      G.bt = swz(tpos);
-      G.gb = base | pos_map<IDX>(tpos, L, m, H);
-
-      float2 a[NSLOT], b[NSLOT];
-#pragma unroll
-      for (int s = 0; s < NSLOT; ++s) {
-        const uint32_t l = G.bt ^ ls_of(G, s);
-        a[s] = tA[l];
-        if constexpr (DUAL) b[s] = tB[l];
-      }
-
-      cw_t w = ph + PS_SIZE;
-      for (uint32_t is = 0; is < nsteps; ++is) {
-        const uint32_t op = w[0] & 0xffffu;
-        const uint32_t nw = w[0] >> 16;
-        if constexpr (MODE == MODE_RED) {
-          // the slot registers do not change in a read-only pass; marking
-          // them as rewritten here keeps the compiler from hoisting the pair
-          // products of all 32 reduction arms out of the step loop
-#pragma unroll
-          for (int s = 0; s < NSLOT; ++s) __asm__ volatile("" : "+v"(a[s].x), "+v"(a[s].y));
-          if (op == OP_RED) red_switch<MODE, IDX>(w[1], a, b, w, G, acc, nred);
-        } else {
-          switch (op) {
-            case OP_ROT: rot_step<MODE, IDX>(a, b, w, G); break;
-            case OP_PH: ph_step<MODE, IDX>(a, b, w, G); break;
-            case OP_SGN: sgn_step<MODE>(a, b, w); break;
-            case OP_TAB: tab_step<MODE>(a, b, w); break;
-            case OP_THR: thr_step<MODE, IDX>(a, b, w, G); break;
-            case OP_SCL: scl_step<MODE>(a, b, w); break;
-            case OP_CXP: cxp_switch<MODE, IDX>(w[1], a, b, w, G); break;
-            case OP_U1: u1_switch<MODE, IDX>(w[1], a, b, w, G); break;
-            case OP_U2:
-              if constexpr (!LEAN) u2_switch<MODE, IDX>(w[1], w[2], a, b, w, G);
-              break;
-            case OP_D2: d2_step<MODE, IDX>(a, b, w, G); break;
-            case OP_PROT:
-              if constexpr (!LEAN) prot_switch<MODE, IDX>(w[1], a, b, w, G);
-              break;
-            case OP_RED:
-              if constexpr (DUAL) red_switch<MODE, IDX>(w[1], a, b, w, G, acc, nred);
-              break;
-            default: break;
-          }
-        }
-        w += nw;
-      }
-
-      if constexpr (WRITE) {
-        // recompute the slot addresses instead of keeping 32 of them live
-        // through the phase
-        uint32_t btw = G.bt;
-        __asm__ volatile("" : "+v"(btw));
-#pragma unroll
-        for (int s = 0; s < NSLOT; ++s) {
-          const uint32_t l = btw ^ ls_of(G, s);
-          tA[l] = a[s];
-          if constexpr (DUAL) tB[l] = b[s];
-        }
-      }
-      __syncthreads();
-      ph = w;
-    }
+    run_phases<TB, MODE, IDX, LEAN>(prog, tA, tB, acc, nred, base, L, m, H);
 
     if constexpr (WRITE) {
       // likewise for the chunk addresses of the tile

@@ -86,7 +86,10 @@ constexpr int kDefaultMaxCost = 400;
 
 // Smallest state that runs as tile passes (2^11 amplitudes: 64 threads x 32).
 constexpr int kPadQubits = 11;
-inline int plan_qubits(int n, bool fuse) { return fuse && n < kPadQubits ? kPadQubits : n; }
+inline int plan_qubits(int n, bool fuse, int min_qubits) {
+  const int target = min_qubits > 0 ? min_qubits : kPadQubits;
+  return fuse && n < target ? target : n;
+}
 
 struct SchedOpts {
   int n = 0;

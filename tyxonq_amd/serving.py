@@ -1,14 +1,18 @@
 """Energy+gradient serving with continuous batching on one GPU.
 
 Requests (qubit count, hardware-efficient ansatz depth and angles, TFIM or
-Heisenberg Hamiltonian) arrive over time.  The server admits a request as
-soon as it has arrived, a slot is free and its adjoint footprint (2^n x 24 B:
-the state, lambda and headroom) fits the memory budget; every admitted
-request runs on a worker with its own HIP stream and resident device states,
-so requests that arrive while others are on the GPU overlap with them.  When
-one finishes, its slot and memory are released and the next request is
-admitted.  `plain_loop` runs the same requests one at a time in arrival order
-on one stream, for comparison.
+Heisenberg Hamiltonian) arrive over time.  Requests of up to 13 qubits are
+served in batched launches: a batch lane takes every small request that has
+arrived and computes all their energies and gradients in one kernel launch,
+one work-group per request with its state and lambda in LDS
+(tqsv_batch_adjoint).  Larger requests are admitted as soon as they have
+arrived, a slot is free and their adjoint footprint (2^n x 24 B: the state,
+lambda and headroom) fits the memory budget; every admitted request runs on a
+worker with its own HIP stream and resident device states, so requests that
+arrive while others are on the GPU overlap with them.  When one finishes, its
+slot and memory are released and the next request is admitted.  `plain_loop`
+runs the same requests one at a time in arrival order on one stream, for
+comparison.
 
 Each request's result depends only on the request: the reductions have a fixed
 order, so a request computes the same energy and gradient alone and in any mix.
@@ -21,7 +25,7 @@ import time
 from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass
 from functools import lru_cache
-from typing import Dict, List, Tuple
+from typing import Dict, List, Optional, Tuple
 
 import numpy as np
 
@@ -142,14 +146,52 @@ def request_coeffs(req: Request) -> np.ndarray:
     return np.concatenate([np.ones(3 * n), np.full(n, g)])
 
 
+# Requests of at most this many qubits run through the batched launch
+# (tqsv_batch_adjoint): one work-group per request, state and lambda in LDS.
+BATCH_MAX_QUBITS = 13
+BATCH_MAX_REQUESTS = 512
+
+
+def is_small(req: Request) -> bool:
+    return req.n <= BATCH_MAX_QUBITS
+
+
 class Cache:
-    """Resources reused across requests of one worker: plans per (n, depth)
-    (a new request only re-emits the programs) and Hamiltonians per
-    (n, kind) (a new request only replaces the coefficients)."""
+    """Resources reused across requests: plans per (n, depth) (a new request
+    only re-emits the programs) and Hamiltonians per (n, kind) (a new request
+    only replaces the coefficients).  Several requests of one shape can be in
+    flight together (a batch), so each key holds a pool."""
 
     def __init__(self):
-        self.plans: Dict[Tuple[int, int], "nat.Plan"] = {}
-        self.hams: Dict[Tuple[int, str], "nat.Ham"] = {}
+        self.plans: Dict[Tuple[int, int], List["nat.Plan"]] = {}
+        self.hams: Dict[Tuple[int, str], List["nat.Ham"]] = {}
+        self.batch: Optional["nat.Batch"] = None
+
+    def acquire(self, req: Request):
+        ops = request_ops(req)
+        pool = self.plans.setdefault((req.n, req.depth), [])
+        if pool:
+            plan = pool.pop()
+            plan.update(ops)
+        else:
+            plan = nat.Plan(req.n, ops, jit=-1, min_qubits=BATCH_MAX_QUBITS if is_small(req) else 0)
+        hpool = self.hams.setdefault((req.n, req.ham), [])
+        if hpool:
+            ham = hpool.pop()
+            ham.set_coeffs(request_coeffs(req))
+        else:
+            x, z, c = PauliSum(req.n, request_terms(req)).to_masks()
+            ham = nat.Ham(req.n, x, z, c)
+        return plan, ham
+
+    def release(self, req: Request, plan, ham) -> None:
+        self.plans[(req.n, req.depth)].append(plan)
+        self.hams[(req.n, req.ham)].append(ham)
+
+    def get_batch(self) -> "nat.Batch":
+        if self.batch is None:
+            self.batch = nat.Batch()
+        return self.batch
 
 
 class _Worker:
@@ -162,29 +204,34 @@ class _Worker:
         self.cache = Cache()
 
 
+def execute_batch(reqs: List[Request], cache: Cache) -> List[Tuple[float, np.ndarray]]:
+    """Energies and gradients of small requests in one launch (current stream)."""
+    got = [cache.acquire(r) for r in reqs]
+    try:
+        psis = [nat.new_state(r.n, 0) for r in reqs]
+        out = cache.get_batch().adjoint([p for p, _ in got], [h for _, h in got], psis,
+                                        [len(r.params) for r in reqs])
+    finally:
+        for r, (p, h) in zip(reqs, got):
+            cache.release(r, p, h)
+    return [(float(e), g) for e, g in out]
+
+
 def execute(req: Request, cache: Cache) -> Tuple[float, np.ndarray]:
-    """Energy and gradient of one request on the current stream."""
+    """Energy and gradient of one request on the current stream.  Small
+    requests take the batched launch (a batch of one), so that a request's
+    result does not depend on whether it was served alone or batched."""
     import torch
 
-    ops = request_ops(req)
-    key = (req.n, req.depth)
-    plan = cache.plans.get(key)
-    if plan is None:
-        plan = nat.Plan(req.n, ops, jit=-1)
-        cache.plans[key] = plan
-    else:
-        plan.update(ops)
-    hk = (req.n, req.ham)
-    ham = cache.hams.get(hk)
-    if ham is None:
-        x, z, c = PauliSum(req.n, request_terms(req)).to_masks()
-        ham = nat.Ham(req.n, x, z, c)
-        cache.hams[hk] = ham
-    else:
-        ham.set_coeffs(request_coeffs(req))
-    psi = nat.new_state(req.n, 0)
-    lam = torch.empty_like(psi)
-    e, g = plan.adjoint(ham, psi, lam, len(req.params))
+    if is_small(req):
+        return execute_batch([req], cache)[0]
+    plan, ham = cache.acquire(req)
+    try:
+        psi = nat.new_state(req.n, 0)
+        lam = torch.empty_like(psi)
+        e, g = plan.adjoint(ham, psi, lam, len(req.params))
+    finally:
+        cache.release(req, plan, ham)
     return float(e), g
 
 
@@ -215,14 +262,51 @@ class Server:
 
     def run(self, reqs: List[Request], realtime: bool = True) -> List[Result]:
         """Serve `reqs` (arrival times relative to the call) and return results
-        in request order."""
+        in request order.  Requests of at most BATCH_MAX_QUBITS qubits go to
+        the batch lane: whenever it is free, it takes every small request that
+        has arrived (up to BATCH_MAX_REQUESTS) and serves them in one launch on
+        its own stream, while larger requests run on the slot workers."""
+        import torch
+
         pending = sorted(reqs, key=lambda r: r.arrival)
         results: Dict[int, Result] = {}
         t0 = time.perf_counter()
         lock = threading.Condition()
         state = {"mem": 0.0, "busy": 0}
+        small_q: List[Tuple[Request, float]] = []
+        lane = {"closed": False}
 
         errors: List[BaseException] = []
+
+        def lane_loop():
+            stream = torch.cuda.Stream()
+            cache = Cache()
+            while True:
+                with lock:
+                    while not small_q and not lane["closed"]:
+                        lock.wait()
+                    if not small_q:
+                        return
+                    batch = small_q[:BATCH_MAX_REQUESTS]
+                    del small_q[:len(batch)]
+                start = time.perf_counter()
+                self.last_stats["batches"] += 1
+                self.last_stats["batched_requests"] += len(batch)
+                try:
+                    with torch.cuda.stream(stream):
+                        out = execute_batch([r for r, _ in batch], cache)
+                except BaseException as exc:  # recorded and re-raised after the run
+                    errors.append(exc)
+                    out = [(float("nan"), np.zeros(0))] * len(batch)
+                done = time.perf_counter()
+                with lock:
+                    for (r, ta), (e, g) in zip(batch, out):
+                        results[r.rid] = Result(r.rid, e, g, ta, start, done)
+                    lock.notify_all()
+
+        self.last_stats = {"batches": 0, "batched_requests": 0}
+        lane_thread = threading.Thread(target=lane_loop, daemon=True)
+        lane_thread.start()
 
         def done_cb(req, fut, t_arr):
             try:
@@ -245,6 +329,12 @@ class Server:
                 if realtime and now < t_arr:
                     time.sleep(min(t_arr - now, 0.002))
                     continue
+                if is_small(req):
+                    with lock:
+                        small_q.append((req, t_arr))
+                        lock.notify_all()
+                    i += 1
+                    continue
                 with lock:
                     while state["busy"] >= self.slots or (
                             state["busy"] > 0 and state["mem"] + self.footprint(req) > self.mem_budget):
@@ -254,6 +344,10 @@ class Server:
                 fut = ex.submit(self._run_one, req)
                 fut.add_done_callback(lambda f, r=req, ta=t_arr: done_cb(r, f, ta))
                 i += 1
+        with lock:
+            lane["closed"] = True
+            lock.notify_all()
+        lane_thread.join()
         if errors:
             raise errors[0]
         return [results[r.rid] for r in reqs]
