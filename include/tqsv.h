@@ -264,6 +264,25 @@ int tqsv_batch_adjoint(tqsv_batch* batch, int nreq, tqsv_plan* const* plans,
                        tqsv_ham* const* hams, const void* const* psi, const int* nparams,
                        double* const* grads, double* energies, void* stream);
 
+/* ---- density matrices ---------------------------------------------------- */
+
+/* A density matrix of n qubits is a 2n-qubit state vec(rho), vec[(r << n) | c]
+ * = rho[r, c]; gates and channels run through the plans on 2n qubits.
+ * Tr(rho P_t) for Pauli terms over the n qubits (masks as tqsv_ham_create)
+ * into terms[t] (host, may be NULL) and sum_t coeff[t] Tr(rho P_t) into *value
+ * (coeff may be NULL: all ones); FP64 reductions.  n <= 16.  Replaces
+ * exp_z_density (libs/quantum_library/kernels/density_matrix.py:66-73) and
+ * DensityMatrixEngine.expval (devices/simulators/density_matrix/engine.py:
+ * 153-183, dense trace of rho H). */
+int tqsv_dm_expval(const void* rho, int n, int nterms, const uint64_t* xmask,
+                   const uint64_t* zmask, const double* coeff, double* terms, double* value,
+                   void* stream);
+
+/* out[r] = sqrt(max(Re rho[r, r], 0)) (an n-qubit complex64 state whose |.|^2
+ * is the diagonal of rho), the input of tqsv_sampler_run for counts from a
+ * density matrix (engine.py:103-151). */
+int tqsv_dm_diag_amplitudes(const void* rho, int n, void* out, void* stream);
+
 /* ---- shot sampling ------------------------------------------------------- */
 
 /* A sampler for n-qubit states: owns the device scratch for up to max_shots

@@ -120,6 +120,8 @@ _SIGS = {
     "tqsv_expval": (_I, [_P, _I, _I, _P, _P, _P, _DP, _DP, _DP, _P]),
     "tqsv_adjoint": (_I, [_P, _P, _P, _P, _I, _DP, _DP, _P]),
     "tqsv_vjp": (_I, [_P, _P, _P, _I, _DP, _P]),
+    "tqsv_dm_expval": (_I, [_P, _I, _I, _P, _P, _P, _P, _DP, _P]),
+    "tqsv_dm_diag_amplitudes": (_I, [_P, _I, _P, _P]),
     "tqsv_batch_create": (_I, [ctypes.POINTER(_P)]),
     "tqsv_batch_destroy": (None, [_P]),
     "tqsv_batch_adjoint": (_I, [_P, _I, _P, _P, _P, _P, _P, _DP, _P]),
@@ -584,4 +586,29 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def dm_expval(vec, n: int, xmask, zmask, coeff=None):
+    """(sum_t coeff_t Tr(rho P_t), [Tr(rho P_t)]) of a vectorised n-qubit
+    density matrix (a 2n-qubit complex64 state on the GPU)."""
+    if _nqubits(vec) != 2 * int(n):
+        raise ValueError("vec must hold a density matrix of n qubits (2n-qubit state)")
+    x = np.ascontiguousarray(np.asarray(xmask, dtype=np.uint64))
+    z = np.ascontiguousarray(np.asarray(zmask, dtype=np.uint64))
+    c = None if coeff is None else np.ascontiguousarray(np.asarray(coeff, dtype=np.float64))
+    terms = np.zeros(max(len(x), 1))
+    val = ctypes.c_double(0.0)
+    _check(load().tqsv_dm_expval(_ptr(vec), int(n), len(x), x.ctypes.data if len(x) else None,
+                                 z.ctypes.data if len(z) else None,
+                                 None if c is None else c.ctypes.data, terms.ctypes.data,
+                                 ctypes.byref(val), stream_ptr()))
+    return val.value, terms[:len(x)]
+
+
+def dm_diag_amplitudes(vec, n: int):
+    """n-qubit complex64 state with |amplitude|^2 = diag(rho)."""
+    torch = _torch()
+    out = torch.empty(1 << int(n), dtype=torch.complex64, device=vec.device)
+    _check(load().tqsv_dm_diag_amplitudes(_ptr(vec), int(n), _ptr(out), stream_ptr()))
+    return out
 

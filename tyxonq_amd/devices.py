@@ -106,14 +106,17 @@ def _select_engine(device: str):
         from .engine import StatevectorEngine
 
         return StatevectorEngine
-    if name in ("simulator:mps", "mps", "matrix_product_state", "simulator:density_matrix",
-                "density_matrix"):
+    if name in ("simulator:density_matrix", "density_matrix"):
+        from .density import DensityMatrixEngine
+
+        return DensityMatrixEngine
+    if name in ("simulator:mps", "mps", "matrix_product_state"):
         raise NotImplementedError(f"simulator device '{device}' is not part of tyxonq_amd")
     raise ValueError(f"Unsupported simulator device: {device}")
 
 
 def list_devices(token: Optional[str] = None, **kws: Any) -> List[str]:
-    return ["simulator::statevector"]
+    return ["simulator::statevector", "simulator::density_matrix"]
 
 
 def _run_one(eng: Any, device: str, c: Any, shots: int, opts: Dict[str, Any]) -> SimTask:
