@@ -104,12 +104,14 @@ def test_compile_native_passes():
     c = tq.Circuit(2).x(0).y(1)
     c.unitary(0, matrix=np.eye(2))
     with pytest.warns(UserWarning, match="auto-added"):
-        low = compile_native(c, {})
+        low, meta = compile_native(c, {})
     names = [op[0] for op in low.ops]
     assert names[:2] == ["rx", "y"]
     assert low.ops[0][2] == pytest.approx(math.pi)
     assert names.count("measure_z") == 2
     assert not hasattr(low, "_unitary_cache")  # dropped by the rebuild, as in the reference
+    assert meta["compile_plan"] == ["rewrite/auto_measure", "rewrite/gates_transform"]
+    assert meta["job_plan"] is None
 
 
 def test_postprocessing_router():

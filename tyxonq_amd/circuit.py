@@ -186,7 +186,7 @@ class Circuit:
         if options:
             opts.update(dict(options))
         opts.update(kw)
-        self._compiled_source = compile_native(self, opts)
+        self._compiled_source, self._compile_meta = compile_native(self, opts)
         return self
 
     # ---- execution ----------------------------------------------------------

@@ -385,6 +385,51 @@ def _apply_factors(psi, factors, dagger: bool) -> None:
             nat.apply_2q(psi, bits[0], bits[1], m)
 
 
+_NO_SHARE = {"kraus", "project_z", "reset", "pulse", "pulse_inline"}
+
+
+def _op_key(op: Any) -> Any:
+    """A hashable, comparable form of an op tuple (tensor and array angles by
+    value); None for ops that must not be shared between circuits."""
+    if not isinstance(op, (list, tuple)) or not op or op[0] in _NO_SHARE:
+        return None
+    key = []
+    for v in op:
+        if hasattr(v, "detach"):
+            a = v.detach().cpu().numpy()
+            key.append(("t", a.shape, a.tobytes()))
+        elif isinstance(v, np.ndarray):
+            key.append(("a", v.shape, v.tobytes()))
+        elif isinstance(v, (list, tuple)):
+            key.append(("l", repr(v)))
+        else:
+            key.append(v)
+    return tuple(key)
+
+
+def _op_keys(circuit: Any) -> List[Any]:
+    return [_op_key(op) for op in circuit.ops]
+
+
+def _common_prefix(a: Sequence[Any], b: Sequence[Any]) -> int:
+    k = 0
+    for x, y in zip(a, b):
+        if x is None or y is None or x != y:
+            break
+        k += 1
+    return k
+
+
+def _with_ops(circuit: Any, ops: List[Any]) -> Any:
+    from .circuit import Circuit
+
+    c = Circuit(int(circuit.num_qubits), ops=list(ops))
+    for attr in ("_unitary_cache", "_kraus_cache"):
+        if hasattr(circuit, attr):
+            setattr(c, attr, getattr(circuit, attr))
+    return c
+
+
 # ---- engine -----------------------------------------------------------------
 
 class StatevectorEngine:
@@ -503,6 +548,68 @@ class StatevectorEngine:
             expectations[f"Z{q}"] = float(val)
         return {"expectations": expectations,
                 "metadata": {"shots": shots, "backend": self.backend.name}}
+
+    def run_many(self, circuits: Sequence[Any], shots: Sequence[int], **kwargs: Any
+                 ) -> List[Dict[str, Any]]:
+        """run() of several circuits with shots > 0 that simulates the ops
+        consecutive circuits have in common once.  Batches such as a base
+        circuit followed by per-group basis rotations (hea_device_runtime.py:
+        150-175) then cost one simulation plus one short suffix per circuit.
+        Circuit i keeps a copy of its state after the ops it shares with
+        circuit i+1 (same width, equal leading ops without kraus or reset type
+        ops), and circuit i+1 starts there.  Each circuit draws its counts as
+        run() does; with ``seed`` circuit i uses the SeedSequence child
+        (seed, i)."""
+        shots = [int(s) for s in shots]
+        if len(shots) != len(circuits):
+            raise ValueError("shots list length must match number of circuits")
+        if any(s <= 0 for s in shots):
+            raise ValueError("run_many samples counts: every shot count must be positive")
+        use_noise = bool(kwargs.get("use_noise", False))
+        noise = (kwargs.get("noise") or {}) if use_noise else None
+        keys = [_op_keys(c) for c in circuits]
+        out: List[Dict[str, Any]] = []
+        snap: Optional[Tuple[int, Any]] = None
+        for i, c in enumerate(circuits):
+            n = int(c.num_qubits)
+            ops = list(c.ops)
+            share = 0
+            if i + 1 < len(circuits) and int(circuits[i + 1].num_qubits) == n:
+                share = _common_prefix(keys[i], keys[i + 1])
+            start, psi = 0, None
+            if snap is not None and i > 0 and int(circuits[i - 1].num_qubits) == n \
+                    and _common_prefix(keys[i - 1], keys[i]) == snap[0]:
+                start, psi = snap
+            snap = None
+            if psi is None:
+                psi = nat.new_state(n, 0)
+            # measure ops of a prefix run by the previous circuit
+            measures = [int(op[1]) for op in ops[:start]
+                        if isinstance(op, (list, tuple)) and op and op[0] == "measure_z"]
+            if share > 0 and share == start:
+                snap = (share, psi.clone())
+            cuts = [start] + ([share] if share > start else []) + [len(ops)]
+            for a, b in zip(cuts[:-1], cuts[1:]):
+                L, m, _ = lower(_with_ops(c, ops[a:b]), mode="run",
+                                reference_dispatch=self.reference_dispatch, track_grad=False,
+                                noise=noise)
+                self._execute(L, psi)
+                measures += m
+                if b == share and share > start:
+                    snap = (share, psi.clone())
+            self.last_state = psi
+            kw = dict(kwargs)
+            if kwargs.get("seed") is not None:
+                kw["seed"] = int(np.random.SeedSequence([int(kwargs["seed"]) & (2 ** 64 - 1), i])
+                                 .generate_state(1, dtype=np.uint64)[0])
+            meta = {"shots": shots[i], "backend": self.backend.name,
+                    "three_level": bool(kwargs.get("three_level", False))}
+            if measures:
+                out.append({"result": self._sample(psi, n, shots[i], kw), "metadata": meta})
+            else:
+                out.append({"expectations": {}, "metadata": meta})
+            del psi
+        return out
 
     def _sample(self, psi, n: int, shots: int, kwargs: Dict[str, Any]) -> Dict[str, int]:
         """Counts over the computational basis (engine.py:377-465) as big-endian
