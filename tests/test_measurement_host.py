@@ -127,3 +127,12 @@ def test_basis_rotation_rows(bases):
         zp = M.z_products(idx, p * 1e6, [tuple((q, P) for P, q in term)], n)[0]
         exact = np.real(np.vdot(psi, D.pauli_sum_dense([(1.0, term)], n) @ psi))
         assert zp == pytest.approx(exact, abs=1e-9)
+
+
+def test_bitstrings_are_big_endian_binary():
+    from tyxonq_amd.engine import bitstrings
+
+    for n in (1, 7, 30, 63):
+        idx = np.random.default_rng(n).integers(0, 2 ** min(n, 62), 40).astype(np.uint64)
+        assert bitstrings(idx, n) == [format(int(i), f"0{n}b") for i in idx]
+    assert bitstrings(np.zeros(0, dtype=np.uint64), 4) == []

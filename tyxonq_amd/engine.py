@@ -385,6 +385,17 @@ def _apply_factors(psi, factors, dagger: bool) -> None:
             nat.apply_2q(psi, bits[0], bits[1], m)
 
 
+def bitstrings(index: Any, n: int) -> List[str]:
+    """Big-endian n-bit strings of basis indices (qubit 0 first), built as
+    one byte array instead of a format call per index."""
+    idx = np.asarray(index, dtype=np.uint64).reshape(-1)
+    if idx.size == 0:
+        return []
+    bits = np.unpackbits(idx.astype(">u8").view(np.uint8).reshape(-1, 8), axis=1)[:, 64 - n:]
+    blob = np.ascontiguousarray(bits + 48).tobytes().decode("ascii")
+    return [blob[i:i + n] for i in range(0, len(blob), n)]
+
+
 _NO_SHARE = {"kraus", "project_z", "reset", "pulse", "pulse_inline"}
 
 
@@ -624,7 +635,7 @@ class StatevectorEngine:
                 seed = int(np.random.SeedSequence().generate_state(1, dtype=np.uint64)[0])
             try:
                 idx, cnt = nat.sample_counts(psi, shots, int(seed))
-                return {format(int(i), f"0{n}b"): int(c) for i, c in zip(idx, cnt)}
+                return dict(zip(bitstrings(idx, n), cnt.tolist()))
             except ValueError:
                 if nat.norm2(psi) > 0.0:
                     raise
