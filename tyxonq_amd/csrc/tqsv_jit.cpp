@@ -12,11 +12,24 @@
 // the program buffer, so the compiled code stays valid when the parameters of
 // a plan change without changing its structure; the generated source is the
 // cache key.
+//
+// The compiler is the hiprtc of the ROCm installation ($ROCM_PATH/lib, else
+// /opt/rocm/lib), loaded into a linker namespace of its own.  A process that
+// imported a framework with a bundled HIP runtime (PyTorch ships its own
+// libhiprtc and libamd_comgr) would otherwise bind hiprtc to that bundle, and
+// with it an older device compiler: the expectation kernels of a 500-term
+// 30-qubit Pauli sum came out of ROCm 7.0's compiler with 206-327 registers
+// and took 60.6 ms, out of ROCm 7.2's with 178-266 and took 43.5 ms.  Only the
+// source-to-code-object step runs in that namespace; code objects are loaded
+// by the process's HIP runtime.  TQSV_HIPRTC=<path> selects another hiprtc,
+// TQSV_HIPRTC=process the one the process resolved.
+#include <dlfcn.h>
 #include <hip/hip_runtime.h>
 #include <hip/hiprtc.h>
 
 #include <algorithm>
 #include <atomic>
+#include <cstdio>
 #include <cstdlib>
 #include <mutex>
 #include <sstream>
@@ -178,32 +191,85 @@ std::string gen_source(int tb, const uint32_t* pw, int n) {
   return o.str();
 }
 
+// the hiprtc entry points the generator uses
+struct Rtc {
+  decltype(&hiprtcCreateProgram) create = nullptr;
+  decltype(&hiprtcCompileProgram) compile = nullptr;
+  decltype(&hiprtcGetProgramLogSize) log_size = nullptr;
+  decltype(&hiprtcGetProgramLog) get_log = nullptr;
+  decltype(&hiprtcGetCodeSize) code_size = nullptr;
+  decltype(&hiprtcGetCode) get_code = nullptr;
+  decltype(&hiprtcDestroyProgram) destroy = nullptr;
+  std::string origin;
+};
+
+template <typename F>
+bool bind(void* h, const char* name, F& f) {
+  f = reinterpret_cast<F>(dlsym(h, name));
+  return f != nullptr;
+}
+
+const Rtc& rtc() {
+  static const Rtc r = [] {
+    Rtc x;
+    std::string path;
+    if (const char* e = std::getenv("TQSV_HIPRTC")) {
+      path = e;
+    } else {
+      const char* root = std::getenv("ROCM_PATH");
+      path = std::string(root && root[0] ? root : "/opt/rocm") + "/lib/libhiprtc.so.7";
+    }
+    if (!path.empty() && path != "process") {
+      if (void* h = dlmopen(LM_ID_NEWLM, path.c_str(), RTLD_NOW | RTLD_LOCAL)) {
+        if (bind(h, "hiprtcCreateProgram", x.create) && bind(h, "hiprtcCompileProgram", x.compile) &&
+            bind(h, "hiprtcGetProgramLogSize", x.log_size) && bind(h, "hiprtcGetProgramLog", x.get_log) &&
+            bind(h, "hiprtcGetCodeSize", x.code_size) && bind(h, "hiprtcGetCode", x.get_code) &&
+            bind(h, "hiprtcDestroyProgram", x.destroy)) {
+          x.origin = path;
+          return x;
+        }
+      }
+    }
+    x.create = &hiprtcCreateProgram;
+    x.compile = &hiprtcCompileProgram;
+    x.log_size = &hiprtcGetProgramLogSize;
+    x.get_log = &hiprtcGetProgramLog;
+    x.code_size = &hiprtcGetCodeSize;
+    x.get_code = &hiprtcGetCode;
+    x.destroy = &hiprtcDestroyProgram;
+    x.origin = "process";
+    return x;
+  }();
+  return r;
+}
+
 bool compile_source(const std::string& src, std::vector<char>& code, std::string& log) {
+  const Rtc& R = rtc();
   hiprtcProgram p;
   const char* hdrs[2] = {kJitHdrDevice, kJitHdrProg};
   const char* names[2] = {"tqsv_device.h", "tqsv_prog.h"};
-  if (hiprtcCreateProgram(&p, src.c_str(), "tqsv_jit_pass.hip", 2, hdrs, names) != HIPRTC_SUCCESS) {
+  if (R.create(&p, src.c_str(), "tqsv_jit_pass.hip", 2, hdrs, names) != HIPRTC_SUCCESS) {
     log = "hiprtcCreateProgram failed";
     return false;
   }
   // no packed FP32 instructions in device code (as for the precompiled kernels)
   const char* opts[] = {"--offload-arch=gfx950", "-O3",          "-std=c++17",       "-Xclang",
                         "-target-feature",       "-Xclang",      "-packed-fp32-ops", "-w"};
-  const hiprtcResult r = hiprtcCompileProgram(p, (int)(sizeof(opts) / sizeof(opts[0])), opts);
+  const hiprtcResult r = R.compile(p, (int)(sizeof(opts) / sizeof(opts[0])), opts);
   size_t ls = 0;
-  hiprtcGetProgramLogSize(p, &ls);
+  R.log_size(p, &ls);
   log.assign(ls, '\0');
-  if (ls) hiprtcGetProgramLog(p, &log[0]);
+  if (ls) R.get_log(p, &log[0]);
   bool ok = r == HIPRTC_SUCCESS;
   if (ok) {
     size_t cs = 0;
-    ok = hiprtcGetCodeSize(p, &cs) == HIPRTC_SUCCESS && cs > 0;
+    ok = R.code_size(p, &cs) == HIPRTC_SUCCESS && cs > 0;
     if (ok) {
       code.resize(cs);
-      ok = hiprtcGetCode(p, code.data()) == HIPRTC_SUCCESS;
+      ok = R.get_code(p, code.data()) == HIPRTC_SUCCESS;
     }
   }
-  hiprtcDestroyProgram(&p);
+  R.destroy(&p);
   return ok;
 }
 
@@ -265,6 +331,15 @@ int jit_prepare(std::vector<PassPlan>& passes, const std::vector<uint32_t>& prog
       TQSV_HIP(hipModuleLoadData(&k.mod, codes[j].data()));
       const uint32_t* pw = prog.data() + passes[todo[j]].prog_off;
       TQSV_HIP(hipModuleGetFunction(&k.fn, k.mod, kernel_name(pass_mode(pw[PH_FLAGS]))));
+      if (const char* v = std::getenv("TQSV_JIT_VERBOSE"); v && v[0] == '1') {
+        // pass shape and the compiler's resource use (scratch = register spills)
+        int regs = 0, scratch = 0;
+        (void)hipFuncGetAttribute(&regs, HIP_FUNC_ATTRIBUTE_NUM_REGS, k.fn);
+        (void)hipFuncGetAttribute(&scratch, HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, k.fn);
+        std::fprintf(stderr, "tqsv jit: pass %zu mode %u L %u m %u phases %u nred %u regs %d scratch %d (hiprtc %s)\n",
+                     todo[j], pass_mode(pw[PH_FLAGS]), pw[PH_L], pw[PH_M], pw[PH_NPHASE], pw[PH_NRED],
+                     regs, scratch, rtc().origin.c_str());
+      }
       g_cache[keys[todo[j]]] = k;
     }
   }
