@@ -853,6 +853,25 @@ __device__ __forceinline__ void jit_load_tile(const float2* __restrict__ psi,
   }
 }
 
+// One 4-byte load per 128-byte line (16 amplitudes) of a tile, two lines per
+// thread: issued for a work-group's next tile while it works on the current
+// one, so that the next tile's loads find their lines in the caches.  The
+// caller keeps the loaded words live to the end of the iteration (an empty asm
+// use), which is where the loads are waited for; barriers only wait for LDS.
+template <int TB, int MODE, typename IDX, typename TM>
+__device__ __forceinline__ void jit_touch_tile(const float2* __restrict__ psi,
+                                               const float2* __restrict__ lam, IDX base,
+                                               uint32_t tid, float (&t)[4]) {
+  const IDX g0 = base | TM::pos((2u * tid) << 4);
+  const IDX g1 = base | TM::pos((2u * tid + 1u) << 4);
+  t[0] = reinterpret_cast<const float*>(psi)[2 * g0];
+  t[1] = reinterpret_cast<const float*>(psi)[2 * g1];
+  if constexpr (MODE == MODE_DUAL) {
+    t[2] = reinterpret_cast<const float*>(lam)[2 * g0];
+    t[3] = reinterpret_cast<const float*>(lam)[2 * g1];
+  }
+}
+
 template <int TB, int MODE, typename IDX, typename TM>
 __device__ __forceinline__ void jit_store_tile(float2* __restrict__ psi, float2* __restrict__ lam,
                                                const float2* tA, const float2* tB, IDX base,
