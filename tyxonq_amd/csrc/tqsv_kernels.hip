@@ -17,6 +17,9 @@ __device__ __forceinline__ uint64_t ins0(uint64_t x, uint32_t b) {
   return ((x >> b) << (b + 1)) | (x & ((1ull << b) - 1ull));
 }
 
+__device__ __forceinline__ uint32_t par(uint32_t v) { return __popc(v) & 1u; }
+__device__ __forceinline__ uint32_t par(uint64_t v) { return __popcll(v) & 1u; }
+
 __device__ __forceinline__ float2 cmulf(float2 a, float2 b) {
   return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
@@ -244,25 +247,37 @@ __global__ __launch_bounds__(BLK) void pauli_dot_terms_kernel(const float2* u, c
   }
 }
 
-__global__ void ham_apply_kernel(const float2* psi, float2* out, uint64_t dim, uint32_t ngroups,
-                                 const uint64_t* gx, const uint32_t* gstart, const uint64_t* tz,
-                                 const float2* tc) {
+// out[i] = sum over flip groups x of c_x(i) psi[i ^ x], c_x(j) = sum_t c_t i^k_t
+// (-1)^{|j & z_t|} over the group's terms; one thread per output amplitude.
+// The group and term arrays are read-only and indexed uniformly across the
+// wave, so they come through scalar loads; the term sign is an XOR of the
+// coefficient's sign bit with the parity of j & z_t (32-bit indices up to 32
+// qubits).
+template <typename IDX>
+__global__ void ham_apply_kernel(const float2* __restrict__ psi, float2* __restrict__ out,
+                                 uint64_t dim, uint32_t ngroups, const uint64_t* __restrict__ gx,
+                                 const uint32_t* __restrict__ gstart,
+                                 const uint64_t* __restrict__ tz, const float2* __restrict__ tc) {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < dim; i += stride) {
+  for (uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i0 < dim; i0 += stride) {
+    const IDX i = (IDX)i0;
     double ar = 0.0, ai = 0.0;  // FP64 across the groups, one rounding at the end
     for (uint32_t g = 0; g < ngroups; ++g) {
-      const uint64_t j = i ^ gx[g];
+      const IDX j = i ^ (IDX)gx[g];
       const float2 v = psi[j];
       float fr = 0.f, fi = 0.f;
-      for (uint32_t t = gstart[g]; t < gstart[g + 1]; ++t) {
-        const float sg = (__popcll(j & tz[t]) & 1) ? -1.f : 1.f;
-        fr += sg * tc[t].x;
-        fi += sg * tc[t].y;
+      const uint32_t t1 = gstart[g + 1];
+#pragma unroll 4
+      for (uint32_t t = gstart[g]; t < t1; ++t) {
+        const uint32_t neg = par((IDX)(j & (IDX)tz[t])) << 31;
+        const float2 c = tc[t];
+        fr += __uint_as_float(__float_as_uint(c.x) ^ neg);
+        fi += __uint_as_float(__float_as_uint(c.y) ^ neg);
       }
       ar += (double)(fr * v.x) - (double)(fi * v.y);
       ai += (double)(fr * v.y) + (double)(fi * v.x);
     }
-    out[i] = make_float2((float)ar, (float)ai);
+    out[i0] = make_float2((float)ar, (float)ai);
   }
 }
 
@@ -387,8 +402,12 @@ hipError_t k_ham_apply(const float2* psi, float2* out, int n, uint32_t ngroups, 
                        const uint32_t* gstart, const uint64_t* tz, const float2* tc,
                        hipStream_t st) {
   const uint64_t dim = 1ull << n;
-  hipLaunchKernelGGL(ham_apply_kernel, dim3(grid_for(dim)), dim3(BLK), 0, st, psi, out, dim,
-                     ngroups, gx, gstart, tz, tc);
+  if (n <= 32)
+    hipLaunchKernelGGL(ham_apply_kernel<uint32_t>, dim3(grid_for(dim)), dim3(BLK), 0, st, psi, out,
+                       dim, ngroups, gx, gstart, tz, tc);
+  else
+    hipLaunchKernelGGL(ham_apply_kernel<uint64_t>, dim3(grid_for(dim)), dim3(BLK), 0, st, psi, out,
+                       dim, ngroups, gx, gstart, tz, tc);
   return hipGetLastError();
 }
 
