@@ -20,6 +20,8 @@ _lib: Optional[ctypes.CDLL] = None
 OP_MAT1 = 1
 OP_MAT2 = 2
 OP_PROT = 3
+# largest state the batched adjoint kernel holds in LDS (state and lambda)
+BATCH_QUBITS = 13
 
 # numpy mirror of struct tqsv_op (include/tqsv.h)
 OP_DTYPE = np.dtype(

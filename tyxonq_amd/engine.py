@@ -24,6 +24,7 @@ ops whose matrices are not in the circuit's caches are skipped.  Pass
 from __future__ import annotations
 
 import math
+import threading
 from collections import OrderedDict
 from typing import Any, Dict, List, Optional, Sequence, Tuple
 
@@ -63,7 +64,7 @@ class _Lowerer:
     def angle(self, theta: Any) -> Tuple[float, int]:
         """(value, param index or -1)."""
         if hasattr(theta, "detach"):
-            val = float(theta.detach().reshape(-1)[0])
+            val = float(theta.item()) if theta.numel() == 1 else float(theta.detach().reshape(-1)[0])
             if self.track_grad and getattr(theta, "requires_grad", False):
                 self.params.append(theta)
                 return val, len(self.params) - 1
@@ -73,7 +74,9 @@ class _Lowerer:
         return float(theta), -1
 
     @staticmethod
-    def _m(mat: np.ndarray) -> List[float]:
+    def _m(mat: Any) -> List[float]:
+        if isinstance(mat, list):  # already the 32 row floats
+            return mat
         flat = np.asarray(mat, dtype=np.complex128).reshape(-1)
         out = np.zeros(32)
         out[0: 2 * flat.size: 2] = flat.real
@@ -103,6 +106,32 @@ class _Lowerer:
             self.rows = []
 
 
+_FIXED_ROWS: Dict[str, List[float]] = {}
+
+
+def _fixed_row(name: str) -> List[float]:
+    """Row floats of an unparameterised gate matrix (built once)."""
+    row = _FIXED_ROWS.get(name)
+    if row is None:
+        mat = {"h": G.gate_h, "x": G.gate_x, "y": G.gate_y, "cz": G.gate_cz_4x4}[name]()
+        row = _FIXED_ROWS[name] = _Lowerer._m(mat)
+    return list(row)
+
+
+def _rot_row(name: str, theta: float) -> List[float]:
+    """Row floats of RX / RY / RZ(theta) (gates.py: cos(t/2) I - i sin(t/2) P,
+    RY = [[c, -s], [s, c]]) without forming the matrix."""
+    t = float(theta)
+    c, s = math.cos(0.5 * t), math.sin(0.5 * t)
+    if name == "rx":
+        head = [c, 0.0, 0.0, -s, 0.0, -s, c, 0.0]
+    elif name == "ry":
+        head = [c, 0.0, -s, 0.0, s, 0.0, c, 0.0]
+    else:
+        head = [c, -s, 0.0, 0.0, 0.0, 0.0, c, s]
+    return head + [0.0] * 24
+
+
 def lower(circuit: Any, *, mode: str = "state", reference_dispatch: bool = True,
           track_grad: bool = False, noise: Optional[Dict[str, Any]] = None):
     """Lower circuit.ops to segments.  Returns (lowerer, measures, z_atten)."""
@@ -129,21 +158,21 @@ def lower(circuit: Any, *, mode: str = "state", reference_dispatch: bool = True,
         if name in skip:
             continue
         if name == "h":
-            L.mat1(op[1], G.gate_h())
+            L.mat1(op[1], _fixed_row("h"))
         elif name in ("rx", "ry", "rz"):
             q = int(op[1])
             th, p = L.angle(op[2])
             b = 1 << L.bit(q)
             if name == "rx":
-                L.mat1(q, G.gate_rx(th), param=p, gx=b)
+                L.mat1(q, _rot_row(name, th), param=p, gx=b)
             elif name == "ry":
-                L.mat1(q, G.gate_ry(th), param=p, gx=b, gz=b)
+                L.mat1(q, _rot_row(name, th), param=p, gx=b, gz=b)
             else:
-                L.mat1(q, G.gate_rz(th), param=p, gz=b)
+                L.mat1(q, _rot_row(name, th), param=p, gz=b)
         elif name in ("cx", "cy"):
             c, t = int(op[1]), int(op[2])
             cm = 1 << L.bit(c)
-            L.mat1(t, G.gate_x() if name == "cx" else G.gate_y(), cmask=cm, cval=cm)
+            L.mat1(t, _fixed_row("x" if name == "cx" else "y"), cmask=cm, cval=cm)
         elif name == "cry":
             c, t = int(op[1]), int(op[2])
             th, p = L.angle(op[3])
@@ -152,7 +181,7 @@ def lower(circuit: Any, *, mode: str = "state", reference_dispatch: bool = True,
             cm = 1 << L.bit(c)
             L.mat1(t, G.gate_ry(th), cmask=cm, cval=cm)
         elif name == "cz":
-            L.mat2(op[1], op[2], G.gate_cz_4x4())
+            L.mat2(op[1], op[2], _fixed_row("cz"))
         elif name == "iswap":
             L.mat2(op[1], op[2], G.gate_iswap_4x4())
         elif name == "swap":
@@ -266,13 +295,14 @@ def _structure_key(n: int, ops: np.ndarray, opts: tuple) -> tuple:
 
 
 def get_plan(n: int, ops: np.ndarray, *, tile_bits: int = 0, max_cost: int = 0,
-             fuse: bool = True) -> nat.Plan:
+             fuse: bool = True, min_qubits: int = 0) -> nat.Plan:
     """A plan for ops, reusing a cached plan of the same structure."""
-    opts = (tile_bits, max_cost, fuse)
+    opts = (tile_bits, max_cost, fuse, min_qubits)
     key = _structure_key(n, ops, opts)
     plan = _PLANS.get(key)
     if plan is None:
-        plan = nat.Plan(n, ops, tile_bits=tile_bits, max_cost=max_cost, fuse=fuse)
+        plan = nat.Plan(n, ops, tile_bits=tile_bits, max_cost=max_cost, fuse=fuse,
+                        min_qubits=min_qubits)
         _PLANS.put(key, plan)
     else:
         plan.update(ops)
@@ -455,6 +485,8 @@ class StatevectorEngine:
         self.tile_bits = tile_bits
         self.max_cost = max_cost
         self.last_state = None  # final state of the last run() (reused by the driver)
+        self._batch = None  # single-launch adjoint for small states (created on first use)
+        self._batch_lock = threading.Lock()
 
     # -- core --
     def _init_state(self, circuit: Any, use_inputs: bool):
@@ -703,8 +735,17 @@ class StatevectorEngine:
                         track_grad=True)
         if len(L.segments) != 1 or L.segments[0][0] != "ops":
             raise NotImplementedError("energy_and_grad needs a purely unitary circuit")
-        plan = self._plan(n, L.segments[0][1])
         psi = self._init_state(circuit, use_inputs=True)
+        if n <= nat.BATCH_QUBITS and not self.tile_bits:
+            # the whole sweep in one launch, state and lambda in LDS
+            plan = get_plan(n, L.segments[0][1], max_cost=self.max_cost, fuse=self.fuse,
+                            min_qubits=nat.BATCH_QUBITS)
+            with self._batch_lock:
+                if self._batch is None:
+                    self._batch = nat.Batch()
+                ((e, g),) = self._batch.adjoint([plan], [ham], [psi], [len(L.params)])
+            return e, g
+        plan = self._plan(n, L.segments[0][1])
         lam = torch.empty_like(psi)
         return plan.adjoint(ham, psi, lam, len(L.params))
 
