@@ -835,7 +835,28 @@ __device__ __forceinline__ void jit_store_slots(const float2 (&a)[NSLOT], const 
   }
 }
 
-template <int TB, int MODE, typename IDX, typename TM>
+// 16-byte global accesses of the tile loops; NT = streaming (non-temporal) hint
+typedef float nt_f4 __attribute__((ext_vector_type(4)));
+template <bool NT>
+__device__ __forceinline__ float4 gload4(const float4* p) {
+  if constexpr (NT) {
+    const nt_f4 v = __builtin_nontemporal_load(reinterpret_cast<const nt_f4*>(p));
+    return make_float4(v.x, v.y, v.z, v.w);
+  } else {
+    return *p;
+  }
+}
+template <bool NT>
+__device__ __forceinline__ void gstore4(float4* p, float4 v) {
+  if constexpr (NT) {
+    const nt_f4 w = {v.x, v.y, v.z, v.w};
+    __builtin_nontemporal_store(w, reinterpret_cast<nt_f4*>(p));
+  } else {
+    *p = v;
+  }
+}
+
+template <int TB, int MODE, typename IDX, typename TM, bool NT = false>
 __device__ __forceinline__ void jit_load_tile(const float2* __restrict__ psi,
                                               const float2* __restrict__ lam, float2* tA,
                                               float2* tB, IDX base, uint32_t tid) {
@@ -847,9 +868,9 @@ __device__ __forceinline__ void jit_load_tile(const float2* __restrict__ psi,
     const uint32_t pk = (uint32_t)k << (TB + 1);
     const IDX g = gthr | TM::pos(pk);
     const uint32_t l = lthr ^ swz(pk);
-    *reinterpret_cast<float4*>(tA + l) = reinterpret_cast<const float4*>(psi)[g >> 1];
+    *reinterpret_cast<float4*>(tA + l) = gload4<NT>(reinterpret_cast<const float4*>(psi) + (g >> 1));
     if constexpr (MODE == MODE_DUAL)
-      *reinterpret_cast<float4*>(tB + l) = reinterpret_cast<const float4*>(lam)[g >> 1];
+      *reinterpret_cast<float4*>(tB + l) = gload4<NT>(reinterpret_cast<const float4*>(lam) + (g >> 1));
   }
 }
 
@@ -872,7 +893,7 @@ __device__ __forceinline__ void jit_touch_tile(const float2* __restrict__ psi,
   }
 }
 
-template <int TB, int MODE, typename IDX, typename TM>
+template <int TB, int MODE, typename IDX, typename TM, bool NT = false>
 __device__ __forceinline__ void jit_store_tile(float2* __restrict__ psi, float2* __restrict__ lam,
                                                const float2* tA, const float2* tB, IDX base,
                                                uint32_t tid) {
@@ -885,9 +906,9 @@ __device__ __forceinline__ void jit_store_tile(float2* __restrict__ psi, float2*
     const uint32_t pk = (uint32_t)k << (TB + 1);
     const IDX g = gthr | TM::pos(pk);
     const uint32_t l = lthr ^ swz(pk);
-    reinterpret_cast<float4*>(psi)[g >> 1] = *reinterpret_cast<const float4*>(tA + l);
+    gstore4<NT>(reinterpret_cast<float4*>(psi) + (g >> 1), *reinterpret_cast<const float4*>(tA + l));
     if constexpr (MODE == MODE_DUAL)
-      reinterpret_cast<float4*>(lam)[g >> 1] = *reinterpret_cast<const float4*>(tB + l);
+      gstore4<NT>(reinterpret_cast<float4*>(lam) + (g >> 1), *reinterpret_cast<const float4*>(tB + l));
   }
 }
 
