@@ -53,7 +53,7 @@ def parse():
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--tile-bits", type=int, default=0)
     ap.add_argument("--max-cost", type=int, default=None,
-                    help="per-pass VALU budget (ops per amplitude); default 200 for the 30q "
+                    help="per-pass VALU budget (ops per amplitude); default 260 for the 30q "
                          "brickwork, the planner's default elsewhere")
     ap.add_argument("--jit", type=int, default=0, help="1 generated kernels, -1 precompiled, 0 auto")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -149,7 +149,7 @@ def run_brickwork(args) -> dict:
     ops = L.segments[0][1]
     t0 = time.perf_counter()
     if args.max_cost is None:
-        args.max_cost = 200
+        args.max_cost = 260
     plan = nat.Plan(n, ops, tile_bits=args.tile_bits, max_cost=args.max_cost, jit=args.jit)
     x, z, c = PauliSum(n, terms).to_masks()
     ham = nat.Ham(n, x, z, c)
