@@ -69,3 +69,29 @@ def test_33_qubit_closed_forms(big_state, jit):
     assert abs(norm - 1.0) < 2e-5
     np.testing.assert_allclose(per, want, atol=2e-5)
     assert abs(e - float(np.sum(want))) < 1e-3
+
+
+def test_33_qubit_sampling_marginals(big_state):
+    import tyxonq_amd as tq
+    from tyxonq_amd import _native as nat
+    from tyxonq_amd.engine import lower
+
+    rng = np.random.default_rng(33)
+    th = rng.uniform(0.2, 2 * np.pi - 0.2, N)
+    c = tq.Circuit(N)
+    for q in range(N):
+        c.ry(q, float(th[q]))
+    for q in range(N - 1):
+        c.cz(q, q + 1)
+    L, _, _ = lower(c)
+    nat.init_basis(big_state, 0)
+    nat.Plan(N, L.segments[0][1]).run(big_state)
+    shots = 200000
+    idx, cnt = nat.sample_counts(big_state, shots, 7)
+    assert int(cnt.sum()) == shots and np.all(np.diff(idx.astype(np.int64)) > 0)
+    assert int(idx.max()) < (1 << N)
+    for q in (0, 1, 15, 31, 32):
+        bit = np.uint64(N - 1 - q)
+        ones = float(cnt[((idx >> bit) & np.uint64(1)).astype(bool)].sum()) / shots
+        p = np.sin(th[q] / 2) ** 2
+        assert abs(ones - p) < 5 * np.sqrt(p * (1 - p) / shots) + 1e-9, (q, ones, p)
