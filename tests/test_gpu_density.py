@@ -112,3 +112,26 @@ def test_driver_routes_density_matrix(gpu):
     assert abs(r["result_meta"]["expectations"]["Z0"]) < 1e-6
     counts = c.run(shots=4000)[0]["result"]
     assert sum(counts.values()) == 4000 and set(counts) <= {"00", "11"}
+
+
+def test_seven_qubit_noisy_circuit(gpu):
+    from tyxonq_amd.density import DensityMatrixEngine
+
+    n = 7
+    rng = np.random.default_rng(77)
+    ops = _random_ops(rng, n, 60)
+    kc = {"ad": DM.amplitude_damping(0.15), "pd": DM.phase_damping(0.3)}
+    ops += [("kraus", 3, "ad"), ("kraus", 6, "pd")] + [("measure_z", q) for q in range(n)]
+    noise = {"type": "pauli", "px": 0.02, "py": 0.01, "pz": 0.03}
+    c = tq.Circuit(n, ops=list(ops))
+    c._kraus_cache = kc
+    eng = DensityMatrixEngine()
+    rho, _ = DM.run(n, ops, noise=noise, kraus_cache=kc)
+    got = eng.density_matrix(c, use_noise=True, noise=noise).cpu().numpy()
+    np.testing.assert_allclose(got, rho, atol=5e-6)
+    out = eng.run(c, shots=0, use_noise=True, noise=noise)
+    for q in range(n):
+        assert abs(out["expectations"][f"Z{q}"] - DM.exp_z(rho, q, n)) < 5e-6
+    terms = [(0.5, [("X", 0), ("X", 6)]), (-1.2, [("Y", 2), ("Z", 3), ("X", 5)]), (0.3, [("Z", 4)])]
+    assert abs(eng.expval(c, terms, use_noise=True, noise=noise)
+               - DM.pauli_sum_expectation(rho, terms, n)) < 5e-6
