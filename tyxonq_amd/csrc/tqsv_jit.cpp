@@ -73,12 +73,6 @@ std::string gen_source(int tb, const uint32_t* pw, int n) {
   const int mode = pass_mode(flags);
   const bool write = (flags & PF_WRITE) != 0;
   const int NT = 1 << tb, TSIZE = 1 << (tb + RB), NWAVE = std::max(NT / 64, 1);
-  // Touch the next tile's lines ahead of its loads in read-only passes over
-  // states larger than the last-level cache (2^25 amplitudes = 256 MiB): there
-  // each tile load waits on HBM latency with little else to do (500-term
-  // 30-qubit expectation 43.4 -> 33.0 ms).  Gate and adjoint passes are bound
-  // by VALU issue or cache-resident, and the extra loads cost them 3-7%.
-  // TQSV_PREFETCH=1 / 0 forces the touches on / off for every pass.
   // Streaming (non-temporal) hints on the tile loads and stores of states
   // larger than the last-level cache, which are read and written once per pass
   // (30-qubit brickwork + 500-term expectation: 134.9 -> 130.0 ms per step).
@@ -86,6 +80,12 @@ std::string gen_source(int tb, const uint32_t* pw, int n) {
   const char* nte = std::getenv("TQSV_NT");
   const int ntm = nte && nte[0] ? std::atoi(nte) : (n >= 25 ? 3 : 0);
   const bool nt_ld = (ntm & 1) != 0, nt_st = (ntm & 2) != 0;
+  // Touch the next tile's lines ahead of its loads in read-only passes over
+  // states larger than the last-level cache (2^25 amplitudes = 256 MiB): there
+  // each tile load waits on HBM latency with little else to do (500-term
+  // 30-qubit expectation 43.4 -> 33.0 ms).  Gate and adjoint passes are bound
+  // by VALU issue or cache-resident, and the extra loads cost them 3-7%.
+  // TQSV_PREFETCH=1 / 0 forces the touches on / off for every pass.
   const char* pfe = std::getenv("TQSV_PREFETCH");
   const bool prefetch = pfe && (pfe[0] == '0' || pfe[0] == '1') ? pfe[0] == '1'
                                                                  : (mode == MODE_RED && n >= 25);
