@@ -240,7 +240,7 @@ int tqsv_plan_create(int n, const tqsv_op* ops, int nops, const tqsv_plan_opts* 
     if (opts) {
       p->so.tile_bits = opts->tile_bits > 0 ? opts->tile_bits : 13;
       p->so.min_low = opts->min_low_bits > 0 ? opts->min_low_bits : 5;
-      p->so.max_cost = opts->max_cost > 0 ? opts->max_cost : kDefaultMaxCost;
+      p->so.max_cost = opts->max_cost > 0 ? opts->max_cost : 0;
       p->so.fuse = opts->fuse != 0;
     }
     p->n = plan_qubits(n, p->so.fuse, opts ? opts->min_qubits : 0);
@@ -304,7 +304,7 @@ int tqsv_plan_inspect(int n, const tqsv_op* ops, int nops, const tqsv_plan_opts*
     if (opts) {
       p.so.tile_bits = opts->tile_bits > 0 ? opts->tile_bits : 13;
       p.so.min_low = opts->min_low_bits > 0 ? opts->min_low_bits : 5;
-      p.so.max_cost = opts->max_cost > 0 ? opts->max_cost : kDefaultMaxCost;
+      p.so.max_cost = opts->max_cost > 0 ? opts->max_cost : 0;
       p.so.fuse = opts->fuse != 0;
     }
     p.n = plan_qubits(n, p.so.fuse, opts ? opts->min_qubits : 0);

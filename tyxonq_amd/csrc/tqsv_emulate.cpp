@@ -393,7 +393,7 @@ extern "C" int tqsv_plan_emulate(int n, const tqsv_op* ops, int nops, const tqsv
     if (opts) {
       p.so.tile_bits = opts->tile_bits > 0 ? opts->tile_bits : 13;
       p.so.min_low = opts->min_low_bits > 0 ? opts->min_low_bits : 5;
-      p.so.max_cost = opts->max_cost > 0 ? opts->max_cost : kDefaultMaxCost;
+      p.so.max_cost = opts->max_cost > 0 ? opts->max_cost : 0;
       p.so.fuse = opts->fuse != 0;
     }
     p.n = plan_qubits(n, p.so.fuse, opts ? opts->min_qubits : 0);

@@ -920,7 +920,7 @@ int select_passes(std::vector<Gate>& gates, const SchedOpts& so, std::vector<Pas
   if (Lmin < T - MAXH) Lmin = T - MAXH;  // at most MAXH high bits fit the pass header
   if (Lmin < 1) Lmin = 1;
   const uint64_t full = lowmask(n);
-  const int budget0 = so.max_cost > 0 ? so.max_cost : kDefaultMaxCost;
+  const int budget0 = so.max_cost > 0 ? so.max_cost : default_max_cost(n);
   // reduction slots per pass: the LDS beyond the tile(s) holds one double per
   // slot and wave
   const size_t lds_tiles = pass_lds_bytes(TBv, mode, 0);

@@ -83,6 +83,11 @@ struct PassPlan {
 // adjoint plans, budgets from 96 up to about 400 cut the time (fewer passes,
 // less per-pass overhead) and larger ones change nothing.
 constexpr int kDefaultMaxCost = 400;
+// States of 2^28+ amplitudes stream from HBM in generated kernels bound by VALU
+// issue; on the 30-qubit brickwork a budget of 260 (21 passes) ran the gate part
+// in 96.4 ms against 98.8 ms at 400 (also 21 passes) and 100.5 ms at 200.
+constexpr int kLargeStateMaxCost = 260;
+inline int default_max_cost(int n) { return n >= 28 ? kLargeStateMaxCost : kDefaultMaxCost; }
 
 // Smallest state that runs as tile passes (2^11 amplitudes: 64 threads x 32).
 constexpr int kPadQubits = 11;
@@ -95,7 +100,7 @@ struct SchedOpts {
   int n = 0;
   int tile_bits = 13;
   int min_low = 5;
-  int max_cost = kDefaultMaxCost;
+  int max_cost = 0;  // 0: default_max_cost(n)
   bool fuse = true;
   bool dual = false;
   bool tile_search = false;  // choose tile bits by admitted work (else in gate order)
