@@ -141,7 +141,8 @@ typedef struct tqsv_op {
 typedef struct tqsv_plan_opts {
   int32_t tile_bits;     /* bits per tile T (11..14); 0 = automatic */
   int32_t min_low_bits;  /* contiguous low bits every tile keeps; 0 = automatic */
-  int32_t max_cost;      /* VALU budget per pass, ops per amplitude; 0 = automatic */
+  int32_t max_cost;      /* VALU budget per pass, ops per amplitude; 0 = automatic
+                            (260 for states of 28+ qubits, 400 below) */
   int32_t fuse;          /* 1 = fused tile passes; 0 = one kernel launch per op */
   int32_t jit;           /* 1 = passes run as kernels generated and compiled for this
                             plan (hiprtc, cached per structure); -1 = the precompiled
