@@ -260,6 +260,11 @@ _H = np.array([[1, 1], [1, -1]], dtype=np.complex128) / math.sqrt(2.0)
 _HSDG = _H @ np.diag([1.0, -1j])
 
 
+# basis-rotation plans by (n, bases), kept apart from the engine's circuit
+# plans (a Hamiltonian of many groups would otherwise push those out)
+_ROT_PLANS: Any = None
+
+
 def basis_rotation_rows(bases: Bases, n: int) -> np.ndarray:
     """Program rows that rotate the X / Y qubits of ``bases`` to Z (H, and
     H S^dag for Y, as hea_device_runtime.py:264-293 appends them)."""
@@ -279,13 +284,22 @@ def basis_rotation_rows(bases: Bases, n: int) -> np.ndarray:
 
 def rotate_to_bases(psi: Any, bases: Bases) -> Any:
     """A copy of psi (GPU complex64) measured in ``bases``."""
+    global _ROT_PLANS
     from . import _native as nat
+    from .engine import _LRU
 
+    if _ROT_PLANS is None:
+        _ROT_PLANS = _LRU(512)
     n = int(psi.numel()).bit_length() - 1
     out = psi.clone()
-    rows = basis_rotation_rows(bases, n)
-    if len(rows):
-        nat.Plan(n, rows).run(out)
+    key = (n, tuple(bases))
+    plan = _ROT_PLANS.get(key)
+    if plan is None:
+        rows = basis_rotation_rows(bases, n)
+        plan = nat.Plan(n, rows) if len(rows) else False
+        _ROT_PLANS.put(key, plan)
+    if plan is not False:
+        plan.run(out)
     return out
 
 
