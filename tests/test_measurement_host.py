@@ -136,3 +136,27 @@ def test_bitstrings_are_big_endian_binary():
         idx = np.random.default_rng(n).integers(0, 2 ** min(n, 62), 40).astype(np.uint64)
         assert bitstrings(idx, n) == [format(int(i), f"0{n}b") for i in idx]
     assert bitstrings(np.zeros(0, dtype=np.uint64), 4) == []
+
+
+def test_shared_prefix_keys():
+    import torch
+
+    from tyxonq_amd.engine import _common_prefix, _op_key, _op_keys
+
+    a = tq.Circuit(3).h(0).rx(1, 0.5).cx(0, 1)
+    b = tq.Circuit(3).h(0).rx(1, 0.5).cz(1, 2)
+    assert _common_prefix(_op_keys(a), _op_keys(b)) == 2
+    # tensor angles compare by value; a different angle ends the prefix
+    t1 = tq.Circuit(1, ops=[("rz", 0, torch.tensor(0.25)), ("h", 0)])
+    t2 = tq.Circuit(1, ops=[("rz", 0, torch.tensor(0.25)), ("x", 0)])
+    t3 = tq.Circuit(1, ops=[("rz", 0, torch.tensor(0.3)), ("h", 0)])
+    assert _common_prefix(_op_keys(t1), _op_keys(t2)) == 1
+    assert _common_prefix(_op_keys(t1), _op_keys(t3)) == 0
+    # stochastic and measurement-type ops are never shared
+    for op in (("kraus", 0, "k"), ("reset", 0), ("project_z", 0, 1)):
+        assert _op_key(op) is None
+    k = [("h", 0), ("kraus", 0, "k"), ("h", 0)]
+    assert _common_prefix([_op_key(o) for o in k], [_op_key(o) for o in k]) == 1
+    # pauli rotations with factor lists and array angles
+    p = ("pauli_rot", [("X", 0), ("Z", 1)], np.float64(0.7))
+    assert _op_key(p) == _op_key(("pauli_rot", [("X", 0), ("Z", 1)], np.float64(0.7)))
