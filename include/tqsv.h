@@ -67,6 +67,14 @@ int tqsv_apply_1q(void* psi, int n, int bit, const double* U, void* stream);
  * and the k<=2 case of apply_kqubit_unitary, statevector.py:71-129. */
 int tqsv_apply_2q(void* psi, int n, int bit_hi, int bit_lo, const double* U, void* stream);
 
+/* out = U psi for a k-qubit matrix U (2^k x 2^k complex, row-major (re, im)
+ * doubles) on bits[0..k-1], bits[0] the most significant index of U; 1 <= k
+ * <= 10, distinct bits; out must be a separate state (psi is not modified).
+ * Replaces apply_kqubit_unitary, statevector.py:71-129, for k >= 3 (used by
+ * the chemistry state preparation, statevector_ops.py:119-137). */
+int tqsv_apply_kq(const void* psi, void* out, int n, int k, const int* bits, const double* U,
+                  void* stream);
+
 /* Apply U (2x2) to `bit` on the amplitudes whose index satisfies
  * (index & ctrl_mask) == ctrl_val.  Controlled gates (cx, cry, cz) of the
  * engine dispatch, engine.py:66-80, without building a 4x4 matrix. */

@@ -152,6 +152,27 @@ def test_kernel_dropins(gpu):
     np.testing.assert_allclose(to_np(out), O.apply_kqubit(v, U, [4, 1], n), atol=2e-6)
 
 
+@pytest.mark.parametrize("qubits", [[0, 1, 2], [5, 2, 7], [3, 0, 6, 1], [7, 4, 0, 2, 5]])
+def test_kqubit_unitary_three_and_more(gpu, qubits):
+    from tyxonq_amd import _native as nat
+    from tyxonq_amd import kernels as K
+
+    n = 8
+    k = len(qubits)
+    rng = np.random.default_rng(10 + k)
+    v = rand_state(rng, n)
+    A = rng.normal(size=(1 << k, 1 << k)) + 1j * rng.normal(size=(1 << k, 1 << k))
+    U = np.linalg.qr(A)[0]
+    src = to_gpu(v)
+    out = K.apply_kqubit_unitary(src, U, qubits, n)
+    np.testing.assert_allclose(to_np(out), O.apply_kqubit(v, U, qubits, n), atol=3e-6)
+    np.testing.assert_array_equal(to_np(src), v.astype(np.complex64))  # input untouched
+    with pytest.raises(ValueError):
+        nat.apply_kq(src, [1, 1, 2], np.eye(8))
+    with pytest.raises(ValueError):
+        nat.apply_kq(src, [0, 1, n], np.eye(8))
+
+
 def test_kraus_and_projection_segments(gpu):
     import tyxonq_amd as tq
 

@@ -96,6 +96,7 @@ _SIGS = {
     "tqsv_init_basis": (_I, [_P, _I, _U64, _P]),
     "tqsv_apply_1q": (_I, [_P, _I, _I, _DP, _P]),
     "tqsv_apply_2q": (_I, [_P, _I, _I, _I, _DP, _P]),
+    "tqsv_apply_kq": (_I, [_P, _P, _I, _I, ctypes.POINTER(ctypes.c_int), _DP, _P]),
     "tqsv_apply_ctrl": (_I, [_P, _I, _U64, _U64, _I, _DP, _P]),
     "tqsv_apply_diag": (_I, [_P, _I, _I, ctypes.POINTER(ctypes.c_int), _DP, _P]),
     "tqsv_pauli_rot": (_I, [_P, _I, _U64, _U64, _D, _P]),
@@ -242,6 +243,21 @@ def apply_2q(psi, bit_hi: int, bit_lo: int, U) -> None:
             _ptr(psi), _nqubits(psi), int(bit_hi), int(bit_lo), mat_buffer(U), stream_ptr()
         )
     )
+
+
+def apply_kq(psi, bits: Sequence[int], U):
+    """U psi for a 2^k x 2^k matrix on index bits `bits` (bits[0] the most
+    significant index of U); returns a new state."""
+    import torch
+
+    k = len(bits)
+    m = np.ascontiguousarray(np.asarray(U, dtype=np.complex128).reshape(1 << k, 1 << k))
+    buf = np.ascontiguousarray(m.view(np.float64).reshape(-1))
+    b = (ctypes.c_int * k)(*[int(x) for x in bits])
+    out = torch.empty_like(psi)
+    _check(load().tqsv_apply_kq(_ptr(psi), _ptr(out), _nqubits(psi), k, b,
+                                buf.ctypes.data_as(_DP), stream_ptr()))
+    return out
 
 
 def apply_ctrl(psi, ctrl_mask: int, ctrl_val: int, bit: int, U) -> None:

@@ -129,6 +129,47 @@ int tqsv_apply_2q(void* psi, int n, int bit_hi, int bit_lo, const double* U, voi
   return 0;
 }
 
+int tqsv_apply_kq(const void* psi, void* out, int n, int k, const int* bits, const double* U,
+                  void* stream) {
+  int rc = check_state(psi, n, "tqsv_apply_kq");
+  if (rc) return rc;
+  if (!out || out == psi) return fail(TQSV_EINVAL, "tqsv_apply_kq: out must be a separate state");
+  if (k < 1 || k > 10 || k > n || !bits || !U)
+    return fail(TQSV_EINVAL, "tqsv_apply_kq: k must be in [1, min(n, 10)]");
+  uint64_t seen = 0;
+  std::vector<uint32_t> pos(k);
+  for (int b = 0; b < k; ++b) {
+    if ((rc = check_bit(bits[b], n, "tqsv_apply_kq"))) return rc;
+    if ((seen >> bits[b]) & 1ull) return fail(TQSV_EINVAL, "tqsv_apply_kq: repeated target bit");
+    seen |= 1ull << bits[b];
+    pos[b] = (uint32_t)bits[b];
+  }
+  const size_t K = (size_t)1 << k;
+  std::vector<uint64_t> off(K, 0);
+  for (size_t j = 0; j < K; ++j)
+    for (int b = 0; b < k; ++b)
+      if ((j >> (k - 1 - b)) & 1u) off[j] |= 1ull << pos[b];
+  std::vector<float2> u(K * K);
+  for (size_t e = 0; e < K * K; ++e) u[e] = make_float2((float)U[2 * e], (float)U[2 * e + 1]);
+  // per-call device copies of the matrix and tables (freed after the launch completes)
+  const size_t bytes = K * K * sizeof(float2) + K * sizeof(uint64_t) + (size_t)k * sizeof(uint32_t);
+  unsigned char* d = nullptr;
+  TQSV_HIP(hipMalloc(reinterpret_cast<void**>(&d), bytes));
+  float2* dU = reinterpret_cast<float2*>(d);
+  uint64_t* dOff = reinterpret_cast<uint64_t*>(d + K * K * sizeof(float2));
+  uint32_t* dPos = reinterpret_cast<uint32_t*>(d + K * K * sizeof(float2) + K * sizeof(uint64_t));
+  hipError_t e = hipMemcpyAsync(dU, u.data(), K * K * sizeof(float2), hipMemcpyHostToDevice, S(stream));
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(dOff, off.data(), K * sizeof(uint64_t), hipMemcpyHostToDevice, S(stream));
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(dPos, pos.data(), (size_t)k * sizeof(uint32_t), hipMemcpyHostToDevice, S(stream));
+  if (e == hipSuccess) e = k_apply_kq(CF(psi), F(out), n, k, dPos, dOff, dU, S(stream));
+  if (e == hipSuccess) e = hipStreamSynchronize(S(stream));
+  (void)hipFree(d);
+  TQSV_HIP(e);
+  return 0;
+}
+
 int tqsv_apply_diag(void* psi, int n, int nbits, const int* bits, const double* phases,
                     void* stream) {
   int rc = check_state(psi, n, "tqsv_apply_diag");

@@ -55,6 +55,8 @@ hipError_t k_project_zero(float2* psi, int n, int bit, int keep, hipStream_t st)
 hipError_t k_scale(float2* psi, int n, float re, float im, hipStream_t st);
 // dst[i] = i < nsrc ? src[i] : 0 for i < ndst (src may be NULL when nsrc = 0)
 hipError_t k_pad_copy(const float2* src, uint64_t nsrc, float2* dst, uint64_t ndst, hipStream_t st);
+hipError_t k_apply_kq(const float2* psi, float2* out, int n, int k, const uint32_t* bitpos,
+                      const uint64_t* off, const float2* U, hipStream_t st);
 hipError_t k_probs(const float2* psi, int n, double* out, hipStream_t st);
 // blockwise FP64 partial sums: mode 0 = |psi|^2, 1 = |psi|^2 (-1)^{bit}, 2 = |psi|^2 [bit==keep]
 uint32_t reduce_grid(int n);

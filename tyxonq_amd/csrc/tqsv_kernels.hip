@@ -281,6 +281,35 @@ __global__ void ham_apply_kernel(const float2* __restrict__ psi, float2* __restr
   }
 }
 
+// out[i] = sum_j U[row(i)][j] psi[base(i) | off[j]] for a k-qubit matrix: row
+// is i's target bits (first target most significant), base is i with those
+// bits cleared, off[j] the target bits of column j.  One thread per output
+// amplitude, FP32 sums (out of place).
+template <typename IDX>
+__global__ void apply_kq_kernel(const float2* __restrict__ psi, float2* __restrict__ out,
+                                uint64_t dim, int k, const uint32_t* __restrict__ bitpos,
+                                const uint64_t* __restrict__ off, const float2* __restrict__ U) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  const uint32_t K = 1u << k;
+  for (uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i0 < dim; i0 += stride) {
+    const IDX i = (IDX)i0;
+    uint32_t row = 0;
+    IDX base = i;
+    for (int b = 0; b < k; ++b) {
+      row = (row << 1) | (uint32_t)((i >> bitpos[b]) & 1u);
+      base &= ~(((IDX)1) << bitpos[b]);
+    }
+    const float2* u = U + (size_t)row * K;
+    float re = 0.f, im = 0.f;
+    for (uint32_t j = 0; j < K; ++j) {
+      const float2 v = psi[base | (IDX)off[j]];
+      re += u[j].x * v.x - u[j].y * v.y;
+      im += u[j].x * v.y + u[j].y * v.x;
+    }
+    out[i0] = make_float2(re, im);
+  }
+}
+
 __global__ void pad_copy_kernel(const float2* __restrict__ src, uint64_t nsrc,
                                 float2* __restrict__ dst, uint64_t ndst) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -293,6 +322,18 @@ hipError_t k_pad_copy(const float2* src, uint64_t nsrc, float2* dst, uint64_t nd
                       hipStream_t st) {
   hipLaunchKernelGGL(pad_copy_kernel, dim3((uint32_t)((ndst + BLK - 1) / BLK)), dim3(BLK), 0, st,
                      src, nsrc, dst, ndst);
+  return hipGetLastError();
+}
+
+hipError_t k_apply_kq(const float2* psi, float2* out, int n, int k, const uint32_t* bitpos,
+                      const uint64_t* off, const float2* U, hipStream_t st) {
+  const uint64_t dim = 1ull << n;
+  if (n <= 32)
+    hipLaunchKernelGGL(apply_kq_kernel<uint32_t>, dim3(grid_for(dim)), dim3(BLK), 0, st, psi, out,
+                       dim, k, bitpos, off, U);
+  else
+    hipLaunchKernelGGL(apply_kq_kernel<uint64_t>, dim3(grid_for(dim)), dim3(BLK), 0, st, psi, out,
+                       dim, k, bitpos, off, U);
   return hipGetLastError();
 }
 

@@ -80,20 +80,24 @@ def apply_2q_statevector(backend: Any, state: Any, gate4: Any, q0: int, q1: int,
 
 def apply_kqubit_unitary(state: Any, unitary: Any, qubit_indices: Sequence[int], num_qubits: int,
                          backend: Any = None):
-    """statevector.py:71-129 for k <= 2: qubit_indices[0] is the most
-    significant index of the unitary."""
+    """statevector.py:71-129: qubit_indices[0] is the most significant index
+    of the unitary; k <= 2 in place on a copy, 3 <= k <= 10 by the k-qubit
+    kernel (out of place)."""
     k = len(qubit_indices)
     if k == 0:
         return state
-    psi = _as_state(state, num_qubits).clone()
     n = int(num_qubits)
+    if k > 10:
+        raise NotImplementedError("apply_kqubit_unitary supports up to 10 target qubits")
+    if k >= 3:
+        return nat.apply_kq(_as_state(state, n), [n - 1 - int(q) for q in qubit_indices],
+                            _mat(unitary, 1 << k))
+    psi = _as_state(state, n).clone()
     if k == 1:
         nat.apply_1q(psi, n - 1 - int(qubit_indices[0]), _mat(unitary, 2))
-    elif k == 2:
+    else:
         nat.apply_2q(psi, n - 1 - int(qubit_indices[0]), n - 1 - int(qubit_indices[1]),
                      _mat(unitary, 4))
-    else:
-        raise NotImplementedError("apply_kqubit_unitary supports k <= 2 on the GPU engine")
     return psi
 
 
