@@ -150,3 +150,16 @@ def test_value_and_grad_cpu_function():
     v, g = vg(np.array([0.3, 1.1]))
     assert v == pytest.approx(math.sin(0.3) ** 2 + math.sin(1.1) ** 2)
     np.testing.assert_allclose(g, [math.sin(0.6), math.sin(2.2)], atol=1e-12)
+
+
+def test_parameter_shift_gradient_convention():
+    from tyxonq_amd.kernels import parameter_shift_gradient
+
+    th = np.array([0.3, -1.1, 2.0])
+    # E = cos t0 * cos t1 + sin t2 (each term generated by a +-1/2 Pauli rotation)
+    f = lambda p: np.cos(p[0]) * np.cos(p[1]) + np.sin(p[2])
+    g = parameter_shift_gradient(f, th)
+    want = [-np.sin(th[0]) * np.cos(th[1]), -np.cos(th[0]) * np.sin(th[1]), np.cos(th[2])]
+    np.testing.assert_allclose(g, want, atol=1e-12)
+    with pytest.raises(ValueError):
+        parameter_shift_gradient(f, np.zeros((2, 2)))
