@@ -82,6 +82,8 @@ __device__ __forceinline__ double half_wave_sum_d(double v) {
   return v;
 }
 
+typedef float f4v __attribute__((ext_vector_type(4)));
+
 __global__ __launch_bounds__(STHR) void seg_sums_kernel(const float2* __restrict__ psi,
                                                         uint32_t blk, double* __restrict__ segsum,
                                                         double* __restrict__ bsum) {
@@ -95,7 +97,9 @@ __global__ __launch_bounds__(STHR) void seg_sums_kernel(const float2* __restrict
     const uint32_t i = w * (SBLK / NW) + r * 128u + 2u * lane;
     double p = 0.0;
     if (i < blk) {
-      const float4 v = *reinterpret_cast<const float4*>(psi + base + i);
+      // streaming load: the state is read once here; the later per-shot reads
+      // of single segments are scattered
+      const f4v v = __builtin_nontemporal_load(reinterpret_cast<const f4v*>(psi + base + i));
       p = prob(make_float2(v.x, v.y)) + prob(make_float2(v.z, v.w));
     }
     const double sg = half_wave_sum_d(p);
