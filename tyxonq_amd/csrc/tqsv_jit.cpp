@@ -90,8 +90,11 @@ std::string gen_source(int tb, const uint32_t* pw, int n) {
   const bool prefetch = pfe && (pfe[0] == '0' || pfe[0] == '1') ? pfe[0] == '1'
                                                                  : (mode == MODE_RED && n >= 25);
   std::ostringstream o;
+  // read-only passes: at least 2 waves per SIMD (at most 256 registers), as
+  // the LDS of two work-groups per CU allows
   o << "#include \"tqsv_device.h\"\nusing namespace tqsv;\n"
-    << "extern \"C\" __global__ void __launch_bounds__(" << NT << ") " << kernel_name(mode)
+    << "extern \"C\" __global__ void __launch_bounds__(" << NT << ") "
+    << (mode == MODE_RED ? "__attribute__((amdgpu_waves_per_eu(2))) " : "") << kernel_name(mode)
     << "(float2* __restrict__ psi, float2* __restrict__ lam, const uint32_t* __restrict__ prog_g,"
        " double* __restrict__ partials, uint32_t ntiles) {\n"
     << "  typedef " << (n > 32 ? "uint64_t" : "uint32_t") << " IDX;\n"
