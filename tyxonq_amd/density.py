@@ -144,7 +144,7 @@ class DensityMatrixEngine:
     def evolve(self, circuit: Any, noise: Optional[Dict[str, Any]] = None) -> Tuple[Any, List[int]]:
         """vec(rho) after the circuit (a 2n-qubit GPU state) and the measured qubits."""
         from .circuit import Circuit
-        from .engine import lower
+        from .engine import get_plan, lower
 
         n = int(circuit.num_qubits)
         vec = nat.new_state(2 * n, 0)
@@ -156,7 +156,8 @@ class DensityMatrixEngine:
 
         def flush():
             if rows:
-                nat.Plan(2 * n, np.array(rows, dtype=nat.OP_DTYPE)).run(vec)
+                # plans cached by structure: a repeated circuit only re-emits its programs
+                get_plan(2 * n, np.array(rows, dtype=nat.OP_DTYPE)).run(vec)
                 rows.clear()
 
         for op in circuit.ops:
