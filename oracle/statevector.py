@@ -391,8 +391,24 @@ def torch_autograd_energy(n: int, build_ops: Callable[[Any], Sequence[tuple]],
 
 # ---- run() expectations with the attenuation noise model ----------------------
 
-_ATTEN_OPS = {"h", "rz", "rx", "ry", "cx", "cry", "cz", "iswap", "swap", "rxx", "ryy", "rzz",
-              "x", "s", "sdg", "unitary"}
+_ATTEN_1W = {"h", "rz", "rx", "ry", "x", "s", "sdg"}                       # engine.py:56-71,100-111
+_ATTEN_2W = {"cx", "cry", "cz", "iswap", "swap", "rxx", "ryy", "rzz"}        # engine.py:72-99
+
+
+def attenuated_wires(op: Sequence[Any], unitary_cache: Optional[Dict[str, Any]] = None
+                     ) -> List[int]:
+    """The qubits whose <Z> run() attenuates after op: the op's named wires
+    only, as in the per-op _attenuate(noise, z_atten, [q]) / [c, t] calls of
+    engine.py:56-141 (never its angle); a unitary op counts only when its
+    matrix is in the cache and it acts on one or two qubits (engine.py:124-141)."""
+    name = op[0]
+    if name in _ATTEN_1W:
+        return [int(op[1])]
+    if name in _ATTEN_2W:
+        return [int(op[1]), int(op[2])]
+    if name == "unitary" and len(op) in (3, 4) and (unitary_cache or {}).get(str(op[-1])) is not None:
+        return [int(a) for a in op[1:-1]]
+    return []
 
 
 def run_expectations(n: int, ops: Sequence[tuple], noise: Optional[Dict[str, Any]] = None,
@@ -407,10 +423,8 @@ def run_expectations(n: int, ops: Sequence[tuple], noise: Optional[Dict[str, Any
     atten = [1.0] * n
     if noise is not None:
         for op in ops:
-            if op[0] in _ATTEN_OPS and op[0] not in _RUN_SKIP:
-                for a in op[1:]:
-                    if isinstance(a, (int, np.integer)) and not isinstance(a, bool):
-                        atten[int(a)] *= fac
+            for q in attenuated_wires(op, unitary_cache):
+                atten[q] *= fac
     return {f"Z{q}": expect_z(psi, q, n) * atten[q] for q in meas}
 
 

@@ -163,3 +163,14 @@ def test_parameter_shift_gradient_convention():
     np.testing.assert_allclose(g, want, atol=1e-12)
     with pytest.raises(ValueError):
         parameter_shift_gradient(f, np.zeros((2, 2)))
+
+
+def test_lowering_attenuates_only_wires():
+    """engine.py:56-141: depolarizing noise attenuates <Z> of the op's wires only."""
+    from tyxonq_amd.circuit import Circuit
+
+    noise = {"type": "depolarizing", "p": 0.3}
+    _, _, z = lower(Circuit(2).rx(0, 1), mode="run", noise=noise)
+    assert z == pytest.approx([0.6, 1.0])
+    _, _, z = lower(Circuit(3).rz(2, 5).cx(0, 1).rzz(1, 2, 3), mode="run", noise=noise)
+    assert z == pytest.approx([0.6, 0.36, 0.36])

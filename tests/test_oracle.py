@@ -281,3 +281,77 @@ def test_gradient_rules_agree():
     assert abs(e_ag - energy(th)) < 1e-12
     np.testing.assert_allclose(ps, ag, atol=1e-10)
     np.testing.assert_allclose(ps, fd, atol=1e-7)
+
+
+# ---- Kraus trajectory (statevector.py:132-218) pinned by closed forms ---------
+
+def _amp_damp(g):
+    return [np.array([[1, 0], [0, math.sqrt(1 - g)]], dtype=complex),
+            np.array([[0, math.sqrt(g)], [0, 0]], dtype=complex)]
+
+
+@pytest.mark.parametrize("g", [0.1, 0.35, 0.8])
+def test_kraus_trajectory_amplitude_damping_on_one(g):
+    """|1>: Born weights (1-g, g); status <= 1-g keeps |1>, above it decays to |0>."""
+    one = np.array([0, 1], dtype=complex)
+    below = O.apply_kraus(one, _amp_damp(g), 0, 1, status=(1 - g) * 0.999)
+    np.testing.assert_allclose(below, [0, 1], atol=1e-14)
+    above = O.apply_kraus(one, _amp_damp(g), 0, 1, status=(1 - g) + 0.5 * g)
+    np.testing.assert_allclose(above, [1, 0], atol=1e-14)
+
+
+@pytest.mark.parametrize("g", [0.2, 0.6])
+def test_kraus_trajectory_bell_state_closed_form(g):
+    """(|00>+|11>)/sqrt2 with damping on qubit 1 (index bit 0): K0 is chosen with
+    probability 1-g/2 and gives (|00> + sqrt(1-g)|11>)/sqrt(2-g); K1 gives |10>
+    (index 2).  A wrong qubit axis, a dropped renormalisation or a selection
+    against the unnormalised weights fails one of the four checks."""
+    bell = np.array([1, 0, 0, 1], dtype=complex) / math.sqrt(2)
+    p0 = 1 - g / 2
+    k0 = O.apply_kraus(bell, _amp_damp(g), 1, 2, status=p0 - 1e-9)
+    want0 = np.array([1, 0, 0, math.sqrt(1 - g)], dtype=complex) / math.sqrt(2 - g)
+    np.testing.assert_allclose(k0, want0, atol=1e-14)
+    k1 = O.apply_kraus(bell, _amp_damp(g), 1, 2, status=p0 + 1e-9)
+    np.testing.assert_allclose(k1, [0, 0, 1, 0], atol=1e-14)
+    # damping on qubit 0 (index bit 1) instead: K1 gives |01> (index 1)
+    k1q0 = O.apply_kraus(bell, _amp_damp(g), 0, 2, status=p0 + 1e-9)
+    np.testing.assert_allclose(k1q0, [0, 1, 0, 0], atol=1e-14)
+    # status beyond every cumulative weight falls back to K0 (selected index 0)
+    np.testing.assert_allclose(O.apply_kraus(bell, _amp_damp(g), 1, 2, status=1.5), want0,
+                               atol=1e-14)
+
+
+def test_kraus_trajectory_unnormalised_kraus_set():
+    """Weights are normalised before selection (statevector.py:189-192): K0 = I/2,
+    K1 = X/2 on |0> have weights 1/4 and 1/4, so status 0.49 picks K0 and 0.51 K1;
+    the result is renormalised either way."""
+    ks = [0.5 * np.eye(2, dtype=complex), 0.5 * O.X]
+    zero = np.array([1, 0], dtype=complex)
+    np.testing.assert_allclose(O.apply_kraus(zero, ks, 0, 1, status=0.49), [1, 0], atol=1e-14)
+    np.testing.assert_allclose(O.apply_kraus(zero, ks, 0, 1, status=0.51), [0, 1], atol=1e-14)
+
+
+def test_kraus_docstring_example():
+    """The reference's own example (statevector.py:168-171): amplitude damping
+    gamma=0.1 on qubit 0 of |00>, status 0.5 -> K0 (weight 1) -> |00>."""
+    zero = np.zeros(4, dtype=complex)
+    zero[0] = 1
+    np.testing.assert_allclose(O.apply_kraus(zero, _amp_damp(0.1), 0, 2, status=0.5), zero,
+                               atol=1e-14)
+
+
+# ---- depolarizing attenuation touches only an op's wires (engine.py:56-141) ----
+
+def test_attenuation_ignores_integer_angles():
+    """rx(0, 1): the angle 1 is not a wire; <Z1> is not attenuated."""
+    noise = {"type": "depolarizing", "p": 0.3}
+    got = O.run_expectations(2, [("rx", 0, 1), ("measure_z", 0), ("measure_z", 1)], noise=noise)
+    assert abs(got["Z0"] - math.cos(1.0) * 0.6) < 1e-12
+    assert abs(got["Z1"] - 1.0) < 1e-12
+    # an integer angle beyond the register must not index a qubit
+    got = O.run_expectations(3, [("rz", 2, 5), ("cx", 0, 1), ("measure_z", 0), ("measure_z", 2)],
+                             noise=noise)
+    assert abs(got["Z0"] - 0.6) < 1e-12 and abs(got["Z2"] - 0.6) < 1e-12
+    # a unitary without a cached matrix is skipped and attenuates nothing
+    got = O.run_expectations(1, [("unitary", 0, "missing"), ("measure_z", 0)], noise=noise)
+    assert abs(got["Z0"] - 1.0) < 1e-12

@@ -390,6 +390,11 @@ class Plan:
             )
         )
 
+    @property
+    def ops(self) -> np.ndarray:
+        """The op rows the plan currently holds (replaced, never mutated, by update)."""
+        return self._ops
+
     def info(self) -> dict:
         inf = PlanInfo()
         _check(load().tqsv_plan_get_info(self._h, ctypes.byref(inf)))

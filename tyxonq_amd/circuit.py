@@ -213,12 +213,17 @@ class Circuit:
         for t in tasks:
             rr = t.get_result(wait=bool(wait_async_result))
             has_payload = (rr.get("result") is not None) or (rr.get("expectations") is not None)
+            # the run's GPU state (shots = 0): analytic post-processing reduces it in place
+            psi_dev = getattr(t.handle, "device_state", None)
             if has_payload:
-                rr["postprocessing"] = apply_postprocessing(rr, dict(self._post_opts))
+                rr["postprocessing"] = apply_postprocessing(rr, dict(self._post_opts),
+                                                            device_state=psi_dev)
                 results.append(rr)
             else:
                 results.append({"result": {}, "result_meta": rr.get("result_meta", {}),
                                 "postprocessing": {"method": None, "result": None}})
+            if psi_dev is not None:
+                t.handle.device_state = None
         # the device layer always returns a task list, so run() returns a list of result
         # dicts, as the reference does (circuit.py:979-981)
         return results

@@ -37,8 +37,9 @@ from .pauli import PauliSum
 
 _STATE_SKIP = {"y", "z", "t", "tdg", "cy", "cry"}
 _RUN_SKIP = {"y", "z", "t", "tdg", "cy"}
-_ATTEN_OPS = {"h", "rz", "rx", "ry", "cx", "cry", "cz", "iswap", "swap", "rxx", "ryy", "rzz",
-              "x", "s", "sdg", "unitary"}
+# ops after which run() attenuates <Z> of their wires under depolarizing noise
+_ATTEN_1W = {"h", "rz", "rx", "ry", "x", "s", "sdg"}
+_ATTEN_2W = {"cx", "cry", "cz", "iswap", "swap", "rxx", "ryy", "rzz"}
 
 
 def _torch():
@@ -257,9 +258,11 @@ def lower(circuit: Any, *, mode: str = "state", reference_dispatch: bool = True,
             raise NotImplementedError("pulse-level ops are not part of the tyxonq_amd engine")
         else:
             continue  # unsupported ops are ignored, as in the reference (engine.py:372-374)
-        if name in _ATTEN_OPS:
-            qs = [a for a in op[1:] if isinstance(a, (int, np.integer))]
-            atten(*qs)
+        if name in _ATTEN_1W:
+            atten(op[1])
+        elif name in _ATTEN_2W or name == "unitary":
+            # the op's wires, never its angle (engine.py:56-141: _attenuate(.., [q]) / [c, t])
+            atten(*op[1:3] if name != "unitary" else op[1:-1])
     L.flush()
     return L, measures, z_atten
 
@@ -333,6 +336,10 @@ def _state_function():
             psi = psi0.clone()
             plan.run(psi)
             ctx.plan = plan
+            # The plan object is shared through the structure cache: a later
+            # circuit of the same structure re-parameterises it in place.  Keep
+            # this circuit's op rows, and restore them before the reverse sweep.
+            ctx.ops = plan.ops
             ctx.meta = [(p.shape, p.dtype, p.device) for p in params]
             ctx.save_for_backward(psi)
             return psi
@@ -342,6 +349,8 @@ def _state_function():
             (psi,) = ctx.saved_tensors
             lam = gpsi.to(torch.complex64).contiguous().clone()
             work = psi.detach().clone()
+            if ctx.plan.ops is not ctx.ops:
+                ctx.plan.update(ctx.ops)
             g = ctx.plan.vjp(work, lam, len(ctx.meta))
             return (None, None) + _grads_like(g, ctx.meta, 1.0)
 

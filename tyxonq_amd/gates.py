@@ -9,9 +9,11 @@ Follows libs/quantum_library/kernels/gates.py of the reference:
     sd = phase(-pi/2), t = phase(pi/4), td = phase(-pi/4)     (gates.py:68-79, 187-200)
 
 Angles are evaluated in FP64; the device receives the matrix rounded once to
-complex64.  The functions also accept 0-dim torch tensors (the value is read
-with ``float``); gradients are produced by the adjoint engine, not by these
-matrices.
+complex64.  A rotation whose angle is a torch tensor that requires grad comes
+back as a complex128 torch tensor built from it (cos/sin of the tensor, as the
+reference builds it, gates.py:19-65), so that the kernel-level drop-ins can
+differentiate through it; any other angle gives a NumPy matrix.  The
+``backend`` arguments are accepted for signature compatibility.
 """
 from __future__ import annotations
 
@@ -43,19 +45,19 @@ def gate_h() -> np.ndarray:
     return np.array([[1, 1], [1, -1]], dtype=_C) / math.sqrt(2.0)
 
 
-def gate_x() -> np.ndarray:
+def gate_x(backend: Any = None) -> np.ndarray:
     return X.copy()
 
 
-def gate_y() -> np.ndarray:
+def gate_y(backend: Any = None) -> np.ndarray:
     return Y.copy()
 
 
-def gate_z() -> np.ndarray:
+def gate_z(backend: Any = None) -> np.ndarray:
     return Z.copy()
 
 
-def gate_phase(theta: Any) -> np.ndarray:
+def gate_phase(theta: Any, backend: Any = None) -> np.ndarray:
     return np.array([[1, 0], [0, np.exp(1j * _f(theta))]], dtype=_C)
 
 
@@ -75,34 +77,52 @@ def gate_td() -> np.ndarray:
     return gate_phase(-math.pi / 4.0)
 
 
-def _rot(P: np.ndarray, theta: Any) -> np.ndarray:
+def _tracked(theta: Any) -> bool:
+    return bool(getattr(theta, "requires_grad", False))
+
+
+def _rot(P: np.ndarray, theta: Any) -> Any:
+    if _tracked(theta):
+        import torch
+
+        t = theta.reshape(()).to(torch.float64)
+        c = torch.cos(0.5 * t).to(torch.complex128)
+        s = torch.sin(0.5 * t).to(torch.complex128)
+        eye = torch.eye(P.shape[0], dtype=torch.complex128, device=t.device)
+        return c * eye + (-1j * s) * torch.as_tensor(P, device=t.device)
     t = _f(theta)
     return math.cos(0.5 * t) * np.eye(P.shape[0], dtype=_C) - 1j * math.sin(0.5 * t) * P
 
 
-def gate_rx(theta: Any) -> np.ndarray:
+def gate_rx(theta: Any, backend: Any = None) -> Any:
     return _rot(X, theta)
 
 
-def gate_ry(theta: Any) -> np.ndarray:
+def gate_ry(theta: Any, backend: Any = None) -> Any:
+    if _tracked(theta):
+        import torch
+
+        t = theta.reshape(()).to(torch.float64)
+        c, s = torch.cos(0.5 * t), torch.sin(0.5 * t)
+        return torch.stack([torch.stack([c, -s]), torch.stack([s, c])]).to(torch.complex128)
     t = _f(theta)
     c, s = math.cos(0.5 * t), math.sin(0.5 * t)
     return np.array([[c, -s], [s, c]], dtype=_C)
 
 
-def gate_rz(theta: Any) -> np.ndarray:
+def gate_rz(theta: Any, backend: Any = None) -> Any:
     return _rot(Z, theta)
 
 
-def gate_rxx(theta: Any) -> np.ndarray:
+def gate_rxx(theta: Any, backend: Any = None) -> Any:
     return _rot(np.kron(X, X), theta)
 
 
-def gate_ryy(theta: Any) -> np.ndarray:
+def gate_ryy(theta: Any, backend: Any = None) -> Any:
     return _rot(np.kron(Y, Y), theta)
 
 
-def gate_rzz(theta: Any) -> np.ndarray:
+def gate_rzz(theta: Any, backend: Any = None) -> Any:
     return _rot(np.kron(Z, Z), theta)
 
 
@@ -132,9 +152,9 @@ def gate_swap_4x4() -> np.ndarray:
     return np.array([[1, 0, 0, 0], [0, 0, 1, 0], [0, 1, 0, 0], [0, 0, 0, 1]], dtype=_C)
 
 
-def gate_cry_4x4(theta: Any) -> np.ndarray:
+def gate_cry_4x4(theta: Any, backend: Any = None) -> np.ndarray:
     m = np.eye(4, dtype=_C)
-    m[2:, 2:] = gate_ry(theta)
+    m[2:, 2:] = gate_ry(_f(theta))
     return m
 
 
