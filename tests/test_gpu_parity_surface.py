@@ -27,7 +27,7 @@ def test_driver_shots0_attaches_state_dispatch_for_cry(gpu):
 
     c = tq.Circuit(3).h(0).cry(0, 1, 1.1).rx(2, 0.4).measure_z(0).measure_z(1)
     (task,) = devices.run(provider="simulator", device="statevector", circuit=c, shots=0)
-    r = task.get_result()
+    r = task.get_result()["result_meta"]  # the driver's payload (driver.py:127-136)
     want_state, _ = O.simulate(3, c.ops, mode="state")
     np.testing.assert_allclose(r["statevector"], want_state, atol=2e-6)
     np.testing.assert_allclose(r["probabilities"], np.abs(want_state) ** 2, atol=2e-6)
@@ -45,7 +45,7 @@ def test_driver_shots0_attaches_state_dispatch_for_inputs(gpu):
     init = rand_state(rng, 4)
     c = tq.Circuit(4, inputs=init).h(1).cx(1, 2).measure_z(2)
     (task,) = devices.run(provider="simulator", device="statevector", circuit=c, shots=0)
-    r = task.get_result()
+    r = task.get_result()["result_meta"]
     want_state, _ = O.simulate(4, c.ops, mode="state", init=init)
     np.testing.assert_allclose(r["statevector"], want_state, atol=2e-6)
     want_run, _ = O.simulate(4, c.ops, mode="run")
@@ -58,7 +58,7 @@ def test_driver_shots0_reuses_run_state_when_dispatches_agree(gpu):
 
     c = tq.Circuit(5).h(0).cx(0, 1).ry(3, 0.3).rzz(2, 4, 0.7).measure_z(1)
     (task,) = devices.run(provider="simulator", device="statevector", circuit=c, shots=0)
-    r = task.get_result()
+    r = task.get_result()["result_meta"]
     want, _ = O.simulate(5, c.ops, mode="state")
     np.testing.assert_allclose(r["statevector"], want, atol=2e-6)
     assert task.handle.device_state is not None and task.handle.device_state.is_cuda

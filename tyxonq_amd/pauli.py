@@ -29,30 +29,45 @@ _CODE = {"I": 0, "X": 1, "Y": 2, "Z": 3}
 _NAME = "IXYZ"
 
 
-def ps2xyz(ps: Sequence[int]) -> Dict[str, List[int]]:
-    xyz: Dict[str, List[int]] = {"x": [], "y": [], "z": []}
+def _code_masks(ps: Sequence[int]) -> Tuple[int, int]:
+    """(x, z) masks over list positions of a code list (bit i = position i):
+    X sets x, Z sets z, Y sets both."""
+    x = z = 0
     for i, v in enumerate(ps):
-        if v == 1:
-            xyz["x"].append(i)
-        elif v == 2:
-            xyz["y"].append(i)
-        elif v == 3:
-            xyz["z"].append(i)
-    return xyz
+        v = int(v)
+        if v in (1, 2):
+            x |= 1 << i
+        if v in (2, 3):
+            z |= 1 << i
+    return x, z
+
+
+def _positions(mask: int) -> List[int]:
+    out = []
+    while mask:
+        low = mask & -mask
+        out.append(low.bit_length() - 1)
+        mask ^= low
+    return out
+
+
+def ps2xyz(ps: Sequence[int]) -> Dict[str, List[int]]:
+    """Code list (0/1/2/3 = I/X/Y/Z per position) -> {"x": [...], "y": [...],
+    "z": [...]} position lists, ascending (kernels/pauli.py:18-31 contract)."""
+    x, z = _code_masks(ps)
+    return {"x": _positions(x & ~z), "y": _positions(x & z), "z": _positions(z & ~x)}
 
 
 def xyz2ps(xyz: Dict[str, List[int]], n: Optional[int] = None) -> List[int]:
-    if n is None:
-        idx = (xyz.get("x", []) + xyz.get("y", []) + xyz.get("z", [])) or [0]
-        n = max(idx) + 1
-    ps = [0] * n
-    for i in xyz.get("x", []):
-        ps[i] = 1
-    for i in xyz.get("y", []):
-        ps[i] = 2
-    for i in xyz.get("z", []):
-        ps[i] = 3
-    return ps
+    """Inverse of ps2xyz; the length defaults to one past the largest position
+    named (1 if none), unnamed positions are I (kernels/pauli.py:34-50
+    contract).  A position named twice takes the later of x, y, z."""
+    code = {}
+    for key, val in (("x", 1), ("y", 2), ("z", 3)):
+        for i in xyz.get(key, []):
+            code[int(i)] = val
+    length = (max(code) + 1 if code else 1) if n is None else int(n)
+    return [code.get(i, 0) for i in range(length)]
 
 
 def pauli_string_to_matrix(ps: Sequence[int]) -> np.ndarray:
@@ -98,15 +113,12 @@ def heisenberg_hamiltonian(num_qubits: int, edges: Sequence[Tuple[int, int]], **
 
 
 def build_tfim_terms(n: int, hzz: float = 1.0, hx: float = -1.0, pbc: bool = False):
-    """hzz * sum Z_i Z_{i+1} + hx * sum X_i (libs/circuits_library/hamiltonians.py:17-26)."""
-    terms: List[Tuple[float, List[Tuple[str, int]]]] = []
-    for i in range(n - 1):
-        terms.append((hzz, [("Z", i), ("Z", i + 1)]))
-    if pbc and n > 1:
-        terms.append((hzz, [("Z", n - 1), ("Z", 0)]))
-    for i in range(n):
-        terms.append((hx, [("X", i)]))
-    return terms
+    """hzz * sum Z_i Z_{i+1} + hx * sum X_i as a term list: the nearest-neighbour
+    bonds first (with the wrap-around bond last when pbc), then the fields, the
+    order of libs/circuits_library/hamiltonians.py:17-26."""
+    bonds = [(i, (i + 1) % n) for i in range(n - 1 + (1 if pbc and n > 1 else 0))]
+    zz = [(hzz, [("Z", a), ("Z", b)]) for a, b in bonds]
+    return zz + [(hx, [("X", q)]) for q in range(n)]
 
 
 def _real(c: Any) -> float:
