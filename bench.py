@@ -65,6 +65,10 @@ def parse():
     ap.add_argument("--requests", type=int, default=120, help="serve workload: requests per rate")
     ap.add_argument("--slots", type=int, default=8, help="serve workload: requests in flight")
     ap.add_argument("--shots", type=int, default=1_000_000, help="sample workload: shots per call")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="default workload: leave out the adjoint and serving parts of the line")
+    ap.add_argument("--serve-rate", type=float, default=200.0,
+                    help="default workload: offered rate of the serving part (req/s)")
     ap.add_argument("--shift-sample", type=int, default=8,
                     help="adjoint workload: parameters timed on the parameter-shift path")
     return ap.parse_args()
@@ -91,6 +95,26 @@ def _threads() -> int:
         return 1
 
 
+def _golden(name: str):
+    """Complex128 results of the same workload (tests/golden/fullsize_c128.json)."""
+    try:
+        with open(os.path.join(ROOT, "tests", "golden", "fullsize_c128.json")) as f:
+            return json.load(f).get(name)
+    except (OSError, ValueError):
+        return None
+
+
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def _traffic(name: str):
     """HBM bytes per launch from a committed rocprofv3 counter summary."""
     path = os.path.join(ROOT, "profiles", f"traffic_{name}.json")
@@ -102,34 +126,61 @@ def _traffic(name: str):
 
 
 # ---- config 2: 30-qubit brickwork + 500-term <H> ----------------------------------
-def cpu_baseline_brickwork(n: int, layers: int) -> dict:
-    """The oracle's per-gate contraction (complex128) on two gates of the
-    workload: RX on the middle qubit and CZ on the middle pair; a layer of the
-    reference's dispatch is 2n one-qubit gates and about n/2 CZs."""
+def cpu_baseline_brickwork(n: int, layers: int, nterms: int) -> dict:
+    """The oracle (NumPy complex128, the reference's per-gate contraction and
+    per-term Pauli application) on a bounded sample of the workload on the
+    host cores: RX (3 runs, median), RZ and CZ on the middle qubits and one
+    2-local term <X_i Y_j>, each on the full 2^n state.  A layer of the
+    reference's dispatch is n RX + n RZ + about n/2 CZ; the job is `layers`
+    layers plus `nterms` terms.  BLAS threads = all host cores."""
+    import contextlib
+
     from oracle import statevector as O
 
-    psi = np.zeros(1 << n, dtype=np.complex128)
-    psi[0] = 1.0
-    q = n // 2
-    t0 = time.perf_counter()
-    psi = O.apply_1q(psi, O.gate_rx(0.3), q, n)
-    t1q = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    psi = O.apply_2q(psi, O.gate_cz(), q, q + 1, n)
-    t2q = time.perf_counter() - t0
-    del psi
-    layer_s = 2 * n * t1q + (n // 2) * t2q
-    job_s = layers * layer_s
+    try:
+        from threadpoolctl import threadpool_limits
+
+        ctx = threadpool_limits(limits=os.cpu_count() or 1)
+    except Exception:
+        ctx = contextlib.nullcontext()
+    with ctx:
+        psi = np.zeros(1 << n, dtype=np.complex128)
+        psi[0] = 1.0
+        q = n // 2
+        t_rx = []
+        for k in range(3):
+            t0 = time.perf_counter()
+            psi = O.apply_1q(psi, O.gate_rx(0.3 + k), q, n)
+            t_rx.append(time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        psi = O.apply_1q(psi, O.gate_rz(0.7), q, n)
+        t_rz = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        psi = O.apply_2q(psi, O.gate_cz(), q, q + 1, n)
+        t_cz = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        O.pauli_sum_expectation(psi, [(1.0, [("X", q - 1), ("Y", q + 2)])], n)
+        t_term = time.perf_counter() - t0
+        del psi
+        threads = _threads()
+    t1 = float(np.median(t_rx))
+    layer_s = n * t1 + n * t_rz + (n // 2) * t_cz
+    job_s = layers * layer_s + nterms * t_term
+    step_bytes = 8 * (1 << n) + layers * 16 * (1 << n) + 8 * (1 << n)
     return {
-        "value": round(layers * 16 * (1 << n) / job_s / 1e9, 3),
+        "value": round(step_bytes / job_s / 1e9, 4),
         "unit": "GB/s",
-        "cores": _threads(),
+        "cores": threads,
         "kind": "port",
-        "sample": (f"NumPy complex128 per-gate contraction (the oracle's restatement of the "
-                   f"reference's apply_1q/apply_2q) timed on 2 gates of the {n}-qubit workload "
-                   f"(RX q{q}: {t1q:.2f} s, CZ q{q},{q + 1}: {t2q:.2f} s), extrapolated to "
-                   f"{layers} layers of {2 * n} one-qubit gates + {n // 2} CZs "
-                   f"({job_s:.0f} s); value counts 16 B/amp per layer like the GPU value"),
+        "cpu": _cpu_model(),
+        "sample": (f"extrapolated: NumPy complex128 per-gate contraction and per-term Pauli "
+                   f"application (the oracle's restatement of apply_1q/apply_2q_statevector and "
+                   f"the Pauli-sum expectation) timed on the full {n}-qubit state: RX q{q} "
+                   f"{t1:.2f} s (median of 3), RZ {t_rz:.2f} s, CZ q{q},{q + 1} {t_cz:.2f} s, one "
+                   f"2-local term {t_term:.2f} s; job = {layers} layers x ({n} RX + {n} RZ + "
+                   f"{n // 2} CZ) + {nterms} terms = {job_s:.0f} s; value counts the same "
+                   f"layer-equivalent bytes as the GPU value"),
+        "job_s": round(job_s, 1),
     }
 
 
@@ -188,6 +239,12 @@ def run_brickwork(args) -> dict:
     achieved = 16 * dim / (per_launch_ms / 1e3) / 1e9
     energy, norm = energies[-1]
     workload = f"brickwork {n}q x {layers} layers + {args.terms}-term 2-local <H>"
+    gold = _golden("brickwork30")
+    vs_c128 = None
+    if gold and (gold["qubits"], gold["layers"], gold["seed"], gold["pauli_terms"]) == \
+            (n, layers, args.seed, args.terms):
+        vs_c128 = {"abs_dE": abs(energy - gold["energy"]), "abs_dnorm2": abs(norm - gold["norm2"]),
+                   "energy_c128": gold["energy"]}
     out = {
         "metric": METRIC,
         "value": round(value, 1),
@@ -216,6 +273,7 @@ def run_brickwork(args) -> dict:
                  "generated_kernels": args.jit >= 0, "build_s": round(build_s, 1)},
         "energy": energy,
         "norm2": norm,
+        "vs_complex128": vs_c128,
         "roofline": {
             "bound": "hbm",
             "achieved": round(achieved, 1),
@@ -239,7 +297,7 @@ def run_brickwork(args) -> dict:
 
 
 # ---- config 3: adjoint energy+gradient vs the plain per-parameter path --------------
-def run_adjoint(args) -> dict:
+def run_adjoint(args, steps=None, warmup=None, n=None, layers=None) -> dict:
     import torch
 
     from tyxonq_amd import _native as nat
@@ -247,8 +305,10 @@ def run_adjoint(args) -> dict:
     from tyxonq_amd.engine import lower
     from tyxonq_amd.pauli import PauliSum
 
-    n = args.n or 22
-    layers = args.layers or 12
+    n = n or args.n or 22
+    layers = layers or args.layers or 12
+    steps = args.steps if steps is None else steps
+    warmup = args.warmup if warmup is None else warmup
     tt = torch.tensor(np.random.default_rng(args.seed).normal(0.0, 0.5, size=2 * n * layers),
                       dtype=torch.float64, requires_grad=True)
     circ, _ = W.ryrz_ring_ansatz(n, layers, params=[tt[i] for i in range(tt.numel())])
@@ -266,16 +326,38 @@ def run_adjoint(args) -> dict:
     plan.adjoint(ham, psi, lam)  # builds the reverse plan
     build_s = time.perf_counter() - t0
     npar = plan.nparams
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         nat.init_basis(psi, 0)
         plan.adjoint(ham, psi, lam)
     _sync()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         nat.init_basis(psi, 0)
         e, g = plan.adjoint(ham, psi, lam)
     _sync()
-    adj_s = (time.perf_counter() - t0) / args.steps
+    adj_s = (time.perf_counter() - t0) / steps
+    # the dominant kernel: the dual pass of the reverse sweep (psi and lambda
+    # read and written once per launch), timed alone through the VJP entry,
+    # which runs exactly the reverse sweep
+    rinfo = plan.info()
+    nrev = max(int(rinfo["reverse_passes"]), 1)
+    ev0, ev1 = _events()
+    vjp_ms = []
+    for _ in range(5):
+        ev0.record()
+        plan.vjp(psi, lam, npar)
+        ev1.record()
+        ev1.synchronize()
+        vjp_ms.append(ev0.elapsed_time(ev1))
+    dual_ms = float(np.median(vjp_ms)) / nrev
+    dual_bytes = 32 * (1 << n)
+    dual_achieved = dual_bytes / (dual_ms * 1e-3) / 1e9
+    gold = _golden("adjoint22")
+    vs_c128 = None
+    if gold and (gold["qubits"], gold["layers"], gold["seed"]) == (n, layers, args.seed):
+        vs_c128 = {"abs_dE": abs(e - gold["energy"]),
+                   "max_abs_dgrad": float(np.max(np.abs(g - np.asarray(gold["grad"])))),
+                   "params_compared": len(gold["grad"])}
 
     # plain path: parameter shift with the same kernels, 2 energies per parameter
     base = ops.copy()
@@ -331,8 +413,8 @@ def run_adjoint(args) -> dict:
         "value": round(1.0 / adj_s, 3),
         "unit": "energy+grad/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
+        "steps": steps,
+        "warmup": warmup,
         "ms_per_step": round(adj_s * 1e3, 3),
         "higher_is_better": True,
         "scaling": "weak",
@@ -352,8 +434,14 @@ def run_adjoint(args) -> dict:
         "speedup_vs_plain": round(plain_s / adj_s, 1),
         "speedup_vs_plain_gpu_only": round(plain_gpu_s / adj_s, 1),
         "max_abs_grad_diff_vs_shift_sample": float(np.max(np.abs(g[:k] - grad_ps))) if k else None,
+        "vs_complex128": vs_c128,
         "build_s": round(build_s, 1),
-        "roofline": None,
+        "roofline": {"bound": "hbm", "achieved": round(dual_achieved, 1), "peak": HBM_PEAK_GBPS,
+                     "unit": "GB/s", "frac": round(dual_achieved / HBM_PEAK_GBPS, 4),
+                     "traffic": None,
+                     "kernel": "tqsv_adjoint_pass (dual pass: psi and lambda read + written)",
+                     "bytes_per_launch": dual_bytes, "avg_launch_ms": round(dual_ms, 5),
+                     "launches_per_sweep": nrev},
         "cpu_baseline": None,
     }
 
@@ -555,6 +643,25 @@ def run_serve(args) -> dict:
     }
 
 
+def serve_at(rate: float, requests: int, slots: int, seed: int) -> dict:
+    """The serving workload at one offered rate: the continuous-batching server
+    and the plain one-at-a-time loop on the same Poisson stream."""
+    from tyxonq_amd import serving as S
+
+    warm = S.make_requests(2 * slots, 1e6, seed=seed + 99)
+    S.Server(slots=slots).run(warm, realtime=False)
+    S.plain_loop(warm[:4], realtime=False)
+    reqs = S.make_requests(requests, rate, seed=seed)
+    server = S.Server(slots=slots)
+    srv = S.summarize(server.run(reqs))
+    srv.update(server.last_stats)
+    plain = S.summarize(S.plain_loop(reqs))
+    return {"rate_rps": rate, "requests": requests, "slots": slots, "server": srv,
+            "plain": plain,
+            "throughput_vs_plain": round(srv["throughput_rps"] / plain["throughput_rps"], 3),
+            "p99_vs_plain": round(srv["p99_ms"] / plain["p99_ms"], 3)}
+
+
 # ---- shot sampling (counts path of engine.run) ------------------------------------
 def run_sample(args) -> dict:
     """Counts of `shots` draws from a 30-qubit state (8 GiB, random amplitudes)
@@ -647,8 +754,15 @@ def main():
         out = run_sample(args)
     else:
         out = run_brickwork(args)
+        if not args.no_extras:
+            # the rest of the metric: config 3 adjoint vs plain, config 5 serving
+            adj = run_adjoint(args, steps=20, warmup=3, n=22, layers=12)
+            out["adjoint_22q"] = {k: adj[k] for k in (
+                "value", "unit", "ms_per_step", "plain_path", "speedup_vs_plain",
+                "speedup_vs_plain_gpu_only", "vs_complex128", "roofline", "config")}
+            out["serve"] = serve_at(args.serve_rate, args.requests, args.slots, args.seed)
         out["cpu_baseline"] = None if args.no_cpu_baseline else cpu_baseline_brickwork(
-            out["config"]["qubits"], out["config"]["layers"])
+            out["config"]["qubits"], out["config"]["layers"], args.terms)
     print(json.dumps(out), flush=True)
 
 

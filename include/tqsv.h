@@ -170,6 +170,8 @@ typedef struct tqsv_plan_info {
   int32_t tile_bits;
   int64_t bytes_per_run;   /* algorithmic HBM bytes: passes * 2 * 8 * 2^n */
   int64_t program_bytes;   /* device program size */
+  int32_t reverse_passes;  /* dual passes of the adjoint / VJP sweep (0 until first built) */
+  int32_t reverse_single;  /* reverse items run as single-op launches */
 } tqsv_plan_info;
 
 typedef struct tqsv_plan tqsv_plan;

@@ -152,3 +152,27 @@ def test_batch_rejects_plans_not_built_for_it(gpu):
     plans[1] = nat.Plan(reqs[1].n, S.request_ops(reqs[1]))  # built for 11 qubits
     with pytest.raises(ValueError):
         nat.Batch().adjoint(plans, hams, psis, [len(r.params) for r in reqs])
+
+
+def test_failed_request_is_isolated_and_others_unchanged(gpu):
+    """A request with a non-finite angle fails alone (host-side validation before
+    any launch); every other request equals its result alone.  A request whose
+    footprint can never fit the budget is rejected at arrival."""
+    reqs = S.make_requests(12, 1e6, seed=21, nmin=10, nmax=16, dmin=2, dmax=4)
+    bad_small = next(r for r in reqs if S.is_small(r))
+    bad_large = next(r for r in reqs if not S.is_small(r))
+    bad_small.params[0] = np.nan
+    bad_large.params[-1] = np.inf
+    huge = S.Request(999, 16, 2, "tfim", 1.0, np.zeros(2 * 16 * 2), 0.0)
+    budget = 24.0 * (1 << 15) * 4
+    mixed = S.Server(slots=4, mem_budget=budget).run(reqs + [huge], realtime=False)
+    good = [r for r in reqs if r.rid not in (bad_small.rid, bad_large.rid)]
+    alone = {r.rid: r for r in S.plain_loop(good, realtime=False)}
+    for req, res in zip(reqs + [huge], mixed):
+        if req.rid in (bad_small.rid, bad_large.rid):
+            assert not res.ok and "non-finite" in res.error
+        elif req.rid == 999 or S.Server.footprint(req) > budget:
+            assert not res.ok and res.error.startswith("rejected")
+        else:
+            assert res.ok and res.energy == alone[req.rid].energy
+            np.testing.assert_array_equal(res.grad, alone[req.rid].grad)

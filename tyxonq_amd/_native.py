@@ -65,6 +65,8 @@ class PlanInfo(ctypes.Structure):
         ("tile_bits", ctypes.c_int32),
         ("bytes_per_run", ctypes.c_int64),
         ("program_bytes", ctypes.c_int64),
+        ("reverse_passes", ctypes.c_int32),
+        ("reverse_single", ctypes.c_int32),
     ]
 
 

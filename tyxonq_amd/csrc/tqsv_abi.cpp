@@ -330,6 +330,10 @@ int tqsv_plan_get_info(const tqsv_plan* p, tqsv_plan_info* info) {
   info->tile_bits = effective_tile_bits(p->n, p->so.tile_bits, false);
   info->bytes_per_run = (int64_t)passes * 16ll * (1ll << p->n);
   info->program_bytes = (int64_t)(p->prog.size() * sizeof(uint32_t));
+  int rt = 0, rs = 0;
+  for (const PassPlan& pp : p->rpasses) (pp.tile ? rt : rs) += 1;
+  info->reverse_passes = rt;
+  info->reverse_single = rs;
   return 0;
 }
 

@@ -11,6 +11,7 @@
 // phases of a pass are carved out of its gates the same way with the register
 // positions as capacity.
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -402,8 +403,12 @@ bool decompose_u1(const cd m[4], RotDec& d, int hint) {
   bool cot = s > c;
   if (hint == 0 && c > 0 && s <= 4.0 * c) cot = false;
   if (hint == 1 && s > 0 && c <= 4.0 * s) cot = true;
-  d.t = cot ? c / s : s / c;
-  d.logk = std::log(g * (cot ? s : c));
+  // The device receives t in float.  The deferred scale is taken from that
+  // rounded t, k = 1/sqrt(1 + t_f^2) (= cos for the tan forms, sin for the cot
+  // forms), so that every applied rotation is exactly unitary and rounding t
+  // only moves the angle (by ~1e-7 rad) instead of the norm.
+  d.t = (double)(float)(cot ? c / s : s / c);
+  d.logk = std::log(g) - 0.5 * std::log1p(d.t * d.t);
   const bool rsign = std::abs(eR.imag()) == 0.0;         // eR = +-1: real form
   const bool isign = !rsign && std::abs(eR.real()) == 0.0;  // eR = +-i: imaginary form
   if (isign) {
@@ -539,12 +544,16 @@ struct PhaseEmitter {
   // (|t| <= 4); every 16 rotations the accumulated real scale is applied.
   // The step is emitted at fixed places, not when the scale gets small, so
   // that the program structure does not depend on the angles.
+  // The factor is rounded to float; what rounding leaves out stays in c0 and
+  // is applied with the next factor (error feedback), so that the norm drift
+  // stays at one rounding however many rescales a plan has.
   void rescale() {
     if (++P.nrot < 16) return;
     const size_t at = e.begin(OP_SCL);
-    e.f((float)std::exp(P.c0.real()));
+    const float f = (float)std::exp(P.c0.real());
+    e.f(f);
     e.end(at);
-    P.c0 = cd(0, P.c0.imag());
+    P.c0 = cd(P.c0.real() - std::log((double)f), P.c0.imag());
     P.nrot = 0;
   }
 
@@ -736,14 +745,24 @@ void emit_phase(std::vector<Gate>& gates, const std::vector<int>& sel,
         break;
       }
       case GK_PROT: {
+        // exp(-i th/2 P) = k (C - i S P) with one of C, S equal to 1 and the
+        // other the tangent (or cotangent) in float; k = +-1/sqrt(1 + t_f^2)
+        // joins the deferred scale, as for OP_ROT, so that each applied
+        // rotation is exactly unitary: c = cos, s = sin in float would change
+        // the norm by up to ~1e-7 per rotation, coherently over the state.
         extract_bits(g.x);
+        const bool tan_form = std::abs(g.s) <= std::abs(g.c);
+        const double tf = (double)(float)(tan_form ? g.s / g.c : g.c / g.s);
+        const double lead = tan_form ? g.c : g.s;
         const size_t at = e.begin(OP_PROT);
         e.u(slotmask(g.x));
         e.u64(g.z);
-        e.f((float)g.c);
-        e.f((float)g.s);
+        e.f(tan_form ? 1.0f : (float)tf);
+        e.f(tan_form ? (float)tf : 1.0f);
         e.u((uint32_t)g.k);
         e.end(at);
+        P.c0 = wrap(P.c0 + cd(-0.5 * std::log1p(tf * tf), lead < 0 ? kPi : 0.0));
+        pe.rescale();
         break;
       }
       case GK_RED: {
@@ -904,6 +923,10 @@ int effective_tile_bits(int n, int requested, bool dual) {
   if (T < 11) T = 11;
   if (T > 14) T = 14;
   if (dual && T > 13) T = 13;
+  if (dual) {
+    // experiment switch: tile bits of dual (adjoint) passes
+    if (const char* e = std::getenv("TQSV_DUAL_T"); e && e[0]) T = std::min(T, std::max(11, std::atoi(e)));
+  }
   if (n > 32) T = 13;  // 64-bit index kernels exist for 2^13-amplitude tiles only
   if (n < T) T = n;
   return T;

@@ -5,14 +5,21 @@ Heisenberg Hamiltonian) arrive over time.  Requests of up to 13 qubits are
 served in batched launches: a batch lane takes every small request that has
 arrived and computes all their energies and gradients in one kernel launch,
 one work-group per request with its state and lambda in LDS
-(tqsv_batch_adjoint).  Larger requests are admitted as soon as they have
-arrived, a slot is free and their adjoint footprint (2^n x 24 B: the state,
-lambda and headroom) fits the memory budget; every admitted request runs on a
-worker with its own HIP stream and resident device states, so requests that
-arrive while others are on the GPU overlap with them.  When one finishes, its
-slot and memory are released and the next request is admitted.  `plain_loop`
-runs the same requests one at a time in arrival order on one stream, for
-comparison.
+(tqsv_batch_adjoint).  Larger requests are admitted in arrival order as soon
+as they have arrived, a slot is free, their adjoint footprint (2^n x 24 B: the
+state, lambda and headroom) fits the memory budget and their share of the
+GPU fits beside the requests already running.  The share of a request is the
+fraction of the chip its tile passes fill: 2^(n-13) work-groups of 2^13
+amplitudes against 512 resident work-groups (2 per CU), capped at 1, so a
+request of 22 or more qubits runs alone (beside the batch lane) while smaller
+ones overlap; running two chip-filling requests at once only time-slices the
+GPU and lengthens both.  Every admitted request runs on a worker with its own
+HIP stream and resident device states.  When one finishes, its slot, memory
+and share are released and the next request is admitted.  A request whose
+footprint can never fit the budget is rejected at arrival, and a request that
+fails (invalid parameters, an error from the library) gets its error in its
+result while the others proceed.  `plain_loop` runs the same requests one at
+a time in arrival order on one stream, for comparison.
 
 Each request's result depends only on the request: the reductions have a fixed
 order, so a request computes the same energy and gradient alone and in any mix.
@@ -54,10 +61,15 @@ class Result:
     arrival: float
     start: float
     done: float
+    error: Optional[str] = None  # set when the request failed or was rejected
 
     @property
     def latency(self) -> float:
         return self.done - self.arrival
+
+    @property
+    def ok(self) -> bool:
+        return self.error is None
 
 
 def make_requests(count: int, rate: float, seed: int = 1234, nmin: int = 10, nmax: int = 26,
@@ -118,9 +130,24 @@ def _ops_template(n: int, depth: int):
     return np.array(rows, dtype=nat.OP_DTYPE), np.array(ry), np.array(rz)
 
 
+def validate(req: Request) -> None:
+    """Host-side checks before anything is launched for a request."""
+    if not (1 <= int(req.n) <= 33):
+        raise ValueError(f"request {req.rid}: {req.n} qubits is outside [1, 33]")
+    if int(req.depth) < 1:
+        raise ValueError(f"request {req.rid}: depth must be positive")
+    th = np.asarray(req.params, dtype=np.float64)
+    if th.size != 2 * req.n * req.depth:
+        raise ValueError(f"request {req.rid}: expected {2 * req.n * req.depth} angles, "
+                         f"got {th.size}")
+    if not np.all(np.isfinite(th)) or not math.isfinite(req.g):
+        raise ValueError(f"request {req.rid}: non-finite angle or field")
+
+
 def request_ops(req: Request) -> np.ndarray:
     """depth x [RY(theta) on every qubit, RZ(theta) on every qubit, CX ladder
     q -> q+1], as bit-level ops with one parameter per rotation."""
+    validate(req)
     tmpl, ry, rz = _ops_template(req.n, req.depth)
     ops = tmpl.copy()
     th = np.asarray(req.params, dtype=np.float64).reshape(req.depth, 2, req.n)
@@ -194,16 +221,6 @@ class Cache:
         return self.batch
 
 
-class _Worker:
-    """Per-thread resources: a HIP stream and a Cache."""
-
-    def __init__(self):
-        import torch
-
-        self.stream = torch.cuda.Stream()
-        self.cache = Cache()
-
-
 def execute_batch(reqs: List[Request], cache: Cache) -> List[Tuple[float, np.ndarray]]:
     """Energies and gradients of small requests in one launch (current stream)."""
     got = [cache.acquire(r) for r in reqs]
@@ -235,29 +252,56 @@ def execute(req: Request, cache: Cache) -> Tuple[float, np.ndarray]:
     return float(e), g
 
 
+def gpu_share(n: int) -> float:
+    """Fraction of the chip the tile passes of an n-qubit request fill."""
+    return min(1.0, float(2 ** max(int(n) - 13, 0)) / 512.0)
+
+
 class Server:
     """Continuous-batching server: up to `slots` requests in flight, admitted
-    in arrival order while their footprint fits `mem_budget` bytes."""
+    in arrival order while their footprint fits `mem_budget` bytes and their
+    GPU shares sum to at most 1.  `execute_fn(req, cache)` and
+    `execute_batch_fn(reqs, cache)` default to the GPU paths; `streams=False`
+    runs without HIP streams (host tests with stand-in executors)."""
 
-    def __init__(self, slots: int = 8, mem_budget: float = 64e9):
+    def __init__(self, slots: int = 8, mem_budget: float = 64e9, execute_fn=None,
+                 execute_batch_fn=None, streams: bool = True):
         self.slots = int(slots)
         self.mem_budget = float(mem_budget)
         self._local = threading.local()
+        self._execute = execute_fn or execute
+        self._execute_batch = execute_batch_fn or execute_batch
+        self._streams = bool(streams)
+        self.last_stats: Dict[str, int] = {}
 
     @staticmethod
     def footprint(req: Request) -> float:
         return float(1 << req.n) * 24.0
 
-    def _run_one(self, req: Request) -> Tuple[float, float, np.ndarray]:
+    def _stream_ctx(self, stream):
+        if stream is None:
+            import contextlib
+
+            return contextlib.nullcontext()
         import torch
 
+        return torch.cuda.stream(stream)
+
+    def _new_stream(self):
+        if not self._streams:
+            return None
+        import torch
+
+        return torch.cuda.Stream()
+
+    def _run_one(self, req: Request) -> Tuple[float, float, np.ndarray]:
         w = getattr(self._local, "w", None)
         if w is None:
-            w = _Worker()
+            w = (self._new_stream(), Cache())
             self._local.w = w
         start = time.perf_counter()
-        with torch.cuda.stream(w.stream):
-            e, g = execute(req, w.cache)
+        with self._stream_ctx(w[0]):
+            e, g = self._execute(req, w[1])
         return start, e, g
 
     def run(self, reqs: List[Request], realtime: bool = True) -> List[Result]:
@@ -265,21 +309,24 @@ class Server:
         in request order.  Requests of at most BATCH_MAX_QUBITS qubits go to
         the batch lane: whenever it is free, it takes every small request that
         has arrived (up to BATCH_MAX_REQUESTS) and serves them in one launch on
-        its own stream, while larger requests run on the slot workers."""
-        import torch
-
+        its own stream, while larger requests run on the slot workers.  A
+        failed request has `error` set in its result; a batch that fails is
+        retried request by request, so one bad request fails alone."""
         pending = sorted(reqs, key=lambda r: r.arrival)
         results: Dict[int, Result] = {}
         t0 = time.perf_counter()
         lock = threading.Condition()
-        state = {"mem": 0.0, "busy": 0}
+        state = {"mem": 0.0, "busy": 0, "share": 0.0}
         small_q: List[Tuple[Request, float]] = []
         lane = {"closed": False}
+        self.last_stats = {"batches": 0, "batched_requests": 0, "rejected": 0, "failed": 0}
 
-        errors: List[BaseException] = []
+        def fail(req: Request, t_arr: float, start: float, exc: BaseException) -> Result:
+            return Result(req.rid, float("nan"), np.zeros(0), t_arr, start, time.perf_counter(),
+                          error=f"{type(exc).__name__}: {exc}")
 
         def lane_loop():
-            stream = torch.cuda.Stream()
+            stream = self._new_stream()
             cache = Cache()
             while True:
                 with lock:
@@ -292,31 +339,42 @@ class Server:
                 start = time.perf_counter()
                 self.last_stats["batches"] += 1
                 self.last_stats["batched_requests"] += len(batch)
+                done_list: List[Result] = []
                 try:
-                    with torch.cuda.stream(stream):
-                        out = execute_batch([r for r, _ in batch], cache)
-                except BaseException as exc:  # recorded and re-raised after the run
-                    errors.append(exc)
-                    out = [(float("nan"), np.zeros(0))] * len(batch)
-                done = time.perf_counter()
+                    with self._stream_ctx(stream):
+                        out = self._execute_batch([r for r, _ in batch], cache)
+                    done = time.perf_counter()
+                    done_list = [Result(r.rid, e, g, ta, start, done)
+                                 for (r, ta), (e, g) in zip(batch, out)]
+                except Exception:
+                    # isolate the failure: every request of the batch alone
+                    for r, ta in batch:
+                        try:
+                            with self._stream_ctx(stream):
+                                ((e, g),) = self._execute_batch([r], cache)
+                            done_list.append(Result(r.rid, e, g, ta, start, time.perf_counter()))
+                        except Exception as exc:
+                            done_list.append(fail(r, ta, start, exc))
                 with lock:
-                    for (r, ta), (e, g) in zip(batch, out):
-                        results[r.rid] = Result(r.rid, e, g, ta, start, done)
+                    for res in done_list:
+                        results[res.rid] = res
+                        self.last_stats["failed"] += 0 if res.ok else 1
                     lock.notify_all()
 
-        self.last_stats = {"batches": 0, "batched_requests": 0}
         lane_thread = threading.Thread(target=lane_loop, daemon=True)
         lane_thread.start()
 
         def done_cb(req, fut, t_arr):
             try:
                 start, e, g = fut.result()
-            except BaseException as exc:  # recorded and re-raised after the run
-                errors.append(exc)
-                start, e, g = t_arr, float("nan"), np.zeros(0)
+                res = Result(req.rid, e, g, t_arr, start, time.perf_counter())
+            except Exception as exc:
+                res = fail(req, t_arr, t_arr, exc)
             with lock:
-                results[req.rid] = Result(req.rid, e, g, t_arr, start, time.perf_counter())
+                results[req.rid] = res
+                self.last_stats["failed"] += 0 if res.ok else 1
                 state["mem"] -= self.footprint(req)
+                state["share"] -= gpu_share(req.n)
                 state["busy"] -= 1
                 lock.notify_all()
 
@@ -329,34 +387,43 @@ class Server:
                 if realtime and now < t_arr:
                     time.sleep(min(t_arr - now, 0.002))
                     continue
+                i += 1
+                if self.footprint(req) > self.mem_budget:
+                    with lock:
+                        results[req.rid] = Result(
+                            req.rid, float("nan"), np.zeros(0), t_arr, t_arr, t_arr,
+                            error=f"rejected: adjoint footprint {self.footprint(req):.3g} B "
+                                  f"exceeds the memory budget {self.mem_budget:.3g} B")
+                        self.last_stats["rejected"] += 1
+                    continue
                 if is_small(req):
                     with lock:
                         small_q.append((req, t_arr))
                         lock.notify_all()
-                    i += 1
                     continue
+                share = gpu_share(req.n)
                 with lock:
                     while state["busy"] >= self.slots or (
-                            state["busy"] > 0 and state["mem"] + self.footprint(req) > self.mem_budget):
+                            state["busy"] > 0 and (
+                                state["mem"] + self.footprint(req) > self.mem_budget
+                                or state["share"] + share > 1.0 + 1e-9)):
                         lock.wait()
                     state["busy"] += 1
                     state["mem"] += self.footprint(req)
+                    state["share"] += share
                 fut = ex.submit(self._run_one, req)
                 fut.add_done_callback(lambda f, r=req, ta=t_arr: done_cb(r, f, ta))
-                i += 1
         with lock:
             lane["closed"] = True
             lock.notify_all()
         lane_thread.join()
-        if errors:
-            raise errors[0]
         return [results[r.rid] for r in reqs]
 
 
-def plain_loop(reqs: List[Request], realtime: bool = True) -> List[Result]:
+def plain_loop(reqs: List[Request], realtime: bool = True, execute_fn=None,
+               sync: bool = True) -> List[Result]:
     """The same requests one at a time, in arrival order, on one stream."""
-    import torch
-
+    run = execute_fn or execute
     cache = Cache()
     out: Dict[int, Result] = {}
     t0 = time.perf_counter()
@@ -366,15 +433,28 @@ def plain_loop(reqs: List[Request], realtime: bool = True) -> List[Result]:
         if realtime and now < t_arr:
             time.sleep(t_arr - now)
         start = time.perf_counter()
-        e, g = execute(req, cache)
-        torch.cuda.current_stream().synchronize()
-        out[req.rid] = Result(req.rid, e, g, t_arr, start, time.perf_counter())
+        try:
+            e, g = run(req, cache)
+            if sync:
+                import torch
+
+                torch.cuda.current_stream().synchronize()
+            out[req.rid] = Result(req.rid, e, g, t_arr, start, time.perf_counter())
+        except Exception as exc:
+            out[req.rid] = Result(req.rid, float("nan"), np.zeros(0), t_arr, start,
+                                  time.perf_counter(), error=f"{type(exc).__name__}: {exc}")
     return [out[r.rid] for r in reqs]
 
 
 def summarize(results: List[Result]) -> Dict[str, float]:
-    lat = np.array([r.latency for r in results])
-    span = max(r.done for r in results) - min(r.arrival for r in results)
-    return {"requests": len(results), "throughput_rps": round(len(results) / span, 2),
-            "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
-            "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3)}
+    """Throughput and latency percentiles over the requests that succeeded."""
+    good = [r for r in results if r.ok]
+    out: Dict[str, float] = {"requests": len(results), "failed": len(results) - len(good)}
+    if not good:
+        return out
+    lat = np.array([r.latency for r in good])
+    span = max(r.done for r in good) - min(r.arrival for r in good)
+    out.update({"throughput_rps": round(len(good) / span, 2),
+                "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
+                "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3)})
+    return out
