@@ -163,7 +163,7 @@ def test_failed_request_is_isolated_and_others_unchanged(gpu):
     bad_large = next(r for r in reqs if not S.is_small(r))
     bad_small.params[0] = np.nan
     bad_large.params[-1] = np.inf
-    huge = S.Request(999, 16, 2, "tfim", 1.0, np.zeros(2 * 16 * 2), 0.0)
+    huge = S.Request(999, 18, 2, "tfim", 1.0, np.zeros(2 * 18 * 2), 0.0)
     budget = 24.0 * (1 << 15) * 4
     mixed = S.Server(slots=4, mem_budget=budget).run(reqs + [huge], realtime=False)
     good = [r for r in reqs if r.rid not in (bad_small.rid, bad_large.rid)]
