@@ -118,6 +118,16 @@ int tqsv_expect_z(const void* psi, int n, int bit, double* out, void* stream);
 int tqsv_dot(const void* u, const void* v, int n, uint64_t xmask, uint64_t zmask, double* out,
              void* stream);
 
+/* tqsv_dot for callers that cannot allocate: code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc.  No counterpart in
+ * the reference.  Same sum as tqsv_dot, written as out_dev[0] (re) and
+ * out_dev[1] (im) into a DEVICE buffer of the caller, without synchronising.
+ * The block partial sums go to a fixed 32 KiB device array of the library
+ * (one per GPU, held until the process ends); nothing is allocated.  Calls
+ * that overlap on one GPU (no stream order between them) race on that array. */
+int tqsv_dot_noalloc(const void* u, const void* v, int n, uint64_t xmask, uint64_t zmask,
+                     double* out_dev, void* stream);
+
 /* probs[i] = |psi_i|^2 as float64 into a DEVICE buffer of 2^n doubles.
  * Replaces StatevectorEngine.probability, engine.py:1041-1047. */
 int tqsv_probs(const void* psi, int n, double* probs_dev, void* stream);

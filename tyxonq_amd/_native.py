@@ -107,6 +107,7 @@ _SIGS = {
     "tqsv_norm2": (_I, [_P, _I, _DP, _P]),
     "tqsv_expect_z": (_I, [_P, _I, _I, _DP, _P]),
     "tqsv_dot": (_I, [_P, _P, _I, _U64, _U64, _DP, _P]),
+    "tqsv_dot_noalloc": (_I, [_P, _P, _I, _U64, _U64, _P, _P]),
     "tqsv_probs": (_I, [_P, _I, _P, _P]),
     "tqsv_plan_create": (_I, [_I, _P, _I, ctypes.POINTER(PlanOpts), ctypes.POINTER(_P)]),
     "tqsv_plan_update": (_I, [_P, _P, _I, _P]),
@@ -326,6 +327,19 @@ def dot(u, v, xmask: int = 0, zmask: int = 0) -> complex:
         raise ValueError("state sizes differ")
     _check(load().tqsv_dot(_ptr(u), _ptr(v), n, int(xmask), int(zmask), out, stream_ptr()))
     return complex(out[0], out[1])
+
+
+def dot_noalloc(u, v, out_dev, xmask: int = 0, zmask: int = 0) -> None:
+    """dot() into out_dev[0:2] (a float64 device tensor), allocating nothing
+    and without synchronising (tqsv_dot_noalloc)."""
+    n = _nqubits(u)
+    if _nqubits(v) != n:
+        raise ValueError("state sizes differ")
+    if (out_dev.numel() < 2 or not out_dev.is_cuda or str(out_dev.dtype) != "torch.float64"
+            or not out_dev.is_contiguous()):
+        raise ValueError("out_dev must be a contiguous float64 device tensor of >= 2 elements")
+    _check(load().tqsv_dot_noalloc(_ptr(u), _ptr(v), n, int(xmask), int(zmask),
+                                   int(out_dev.data_ptr()), stream_ptr()))
 
 
 def probs(psi):
