@@ -194,3 +194,56 @@ def test_small_states_run_padded(n):
         info = nat.inspect(n, lower(tq.Circuit(n, ops=list(ops)), mode="run",
                                     reference_dispatch=False)[0].segments[0][1])
         assert info["n"] == n and info["passes"] <= 4
+
+
+def _commuting_group(rng, n, x_qubits, size):
+    """`size` Pauli strings on x_qubits with X/Y letters whose Y counts all
+    have one parity (so they commute pairwise) and a shared Z string outside."""
+    zq = [q for q in range(n) if q not in x_qubits][: int(rng.integers(0, 4))]
+    par = int(rng.integers(2))
+    out, seen = [], set()
+    while len(out) < size:
+        letters = tuple("XY"[int(rng.integers(2))] for _ in x_qubits)
+        if letters.count("Y") % 2 != par or letters in seen:
+            continue
+        seen.add(letters)
+        out.append(tuple(zip(letters, x_qubits)) + tuple(("Z", q) for q in zq))
+    return out
+
+
+@pytest.mark.parametrize("width,size", [(2, 2), (4, 8), (3, 4), (5, 16)])
+def test_grouped_commuting_rotations_emulated(width, size):
+    """Runs of commuting rotations with one flip mask become one grouped step
+    (a two-level rotation per pair class); compared with the oracle's
+    rotation-by-rotation product, interleaved with gates that break runs."""
+    n = 13
+    rng = np.random.default_rng(10 * width + size)
+    ops = []
+    for _ in range(6):
+        xq = [int(q) for q in rng.choice(n, width, replace=False)]
+        for fac in _commuting_group(rng, n, xq, size):
+            ops.append(("pauli_rot", fac, float(rng.normal(0, 0.7))))
+        ops.append(("ry", int(rng.integers(n)), 0.4))
+    v = rand_state(rng, n)
+    want, _ = O.simulate(n, ops, mode="run", init=v, reference_dispatch=False)
+    np.testing.assert_allclose(_emulated(n, ops, v), want, atol=2e-5)
+    info = nat.inspect(n, lower(tq.Circuit(n, ops=ops), mode="run",
+                                reference_dispatch=False)[0].segments[0][1])
+    assert info["gates_after_merge"] <= 6 * 2 + 1  # each run became one gate
+
+
+def test_ucc_circuit_emulated():
+    from tyxonq_amd import chem
+
+    ns, no = 5, 2
+    n = 2 * ns
+    ex, ids = chem.uccsd_excitations(no, ns - no)
+    th = np.random.default_rng(3).normal(0, 0.3, size=max(ids) + 1)
+    ans = chem.UCCAnsatz(n, (no, no), ex, ids)
+    c = chem.ucc_circuit(list(th), n, (no, no), ex, ids)
+    want, _ = O.simulate(n, c.ops)
+    v = np.zeros(1 << n, dtype=np.complex64)
+    v[0] = 1
+    got = nat.emulate(n, ans.ops(th), v)
+    np.testing.assert_allclose(got, want, atol=2e-6)
+    assert nat.inspect(n, ans.ops(th))["gates_after_merge"] <= len(ex) + 2 * no

@@ -586,6 +586,63 @@ __device__ __forceinline__ void prot_x(float2 (&a)[NSLOT], float2 (&b)[NSLOT], c
 #undef TQSV_PR
 }
 
+// ---- OP_PGRP ------------------------------------------------------------
+// compressed index of slot s over the slot bits of mask M (ascending)
+template <uint32_t M>
+__device__ __forceinline__ constexpr uint32_t pext_slot(uint32_t s) {
+  uint32_t r = 0, k = 0;
+  for (int b = 0; b < RB; ++b)
+    if ((M >> b) & 1u) {
+      if ((s >> b) & 1u) r |= 1u << k;
+      ++k;
+    }
+  return r;
+}
+
+// A group of commuting Pauli rotations on the pairs (s, s ^ XR): per pair
+// class the two-level rotation new_i = a_i + (cm1 a_i + sg(j) u_j a_j),
+// new_j = a_j + (cm1 a_j + sg(i) u_i a_i), the small bracket first so that
+// each amplitude is rounded once at its own size.  ACT >= 0: the active class
+// mask fixed at compile time (generated kernels); ACT < 0: read at run time.
+template <int XR, int ACT, typename IDX>
+__device__ __forceinline__ void pgrp_apply1(float2 (&a)[NSLOT], const PhaseGeo<IDX>& G, IDX z0,
+                                            uint32_t kpar, uint32_t act, cw_t tab) {
+  constexpr int LB = XR & (-XR);
+  constexpr uint32_t HI = (uint32_t)XR & ~(uint32_t)LB;
+  const uint32_t tp = parity((IDX)(G.gb & z0));
+#pragma unroll
+  for (int s = 0; s < NSLOT; ++s) {
+    if (s & LB) continue;
+    const uint32_t cls = pext_slot<HI>((uint32_t)s);
+    if (ACT >= 0 ? !(((uint32_t)ACT >> cls) & 1u) : !((act >> cls) & 1u)) continue;
+    const int s1 = s ^ XR;
+    const cw_t e = tab + PG_ENT * cls;
+    const float cm1 = fl(e);
+    const uint32_t sgi = tp ^ parity((IDX)(gs_of(G, s) & z0));
+    const uint32_t sgj = sgi ^ kpar;
+    const float uir = sgi ? -fl(e + 1) : fl(e + 1), uii = sgi ? -fl(e + 2) : fl(e + 2);
+    const float ujr = sgj ? -fl(e + 3) : fl(e + 3), uji = sgj ? -fl(e + 4) : fl(e + 4);
+    const float2 x0 = a[s], x1 = a[s1];
+    const float t0r = fmaf(-uji, x1.y, fmaf(cm1, x0.x, ujr * x1.x));
+    const float t0i = fmaf(uji, x1.x, fmaf(cm1, x0.y, ujr * x1.y));
+    const float t1r = fmaf(-uii, x0.y, fmaf(cm1, x1.x, uir * x0.x));
+    const float t1i = fmaf(uii, x0.x, fmaf(cm1, x1.y, uir * x0.y));
+    a[s] = make_float2(x0.x + t0r, x0.y + t0i);
+    a[s1] = make_float2(x1.x + t1r, x1.y + t1i);
+  }
+}
+
+template <int XR, int MODE, typename IDX>
+__device__ __forceinline__ void pgrp_x(float2 (&a)[NSLOT], float2 (&b)[NSLOT], cw_t w,
+                                       const PhaseGeo<IDX>& G) {
+  if constexpr (XR != 0) {
+    const IDX z0 = ld_idx<IDX>(w + 2);
+    const uint32_t kpar = w[4], act = w[5];
+    pgrp_apply1<XR, -1, IDX>(a, G, z0, kpar, act, w + PG_HDR);
+    if constexpr (MODE == MODE_DUAL) pgrp_apply1<XR, -1, IDX>(b, G, z0, kpar, act, w + PG_HDR);
+  }
+}
+
 // ---- OP_RED -------------------------------------------------------------
 // +-1.0 from bit s of a uniform mask, built with scalar bit operations
 __device__ __forceinline__ float sgn1(uint32_t mask, int s) {
@@ -700,6 +757,19 @@ __device__ __forceinline__ void prot_switch(uint32_t xr, float2 (&a)[NSLOT], flo
       return;
     }
     prot_switch<MODE, IDX, X + 1>(xr, a, b, w, G);
+  }
+}
+
+template <int MODE, typename IDX, int X = 0>
+__device__ __forceinline__ void pgrp_switch(uint32_t xr, float2 (&a)[NSLOT], float2 (&b)[NSLOT],
+                                            cw_t w, const PhaseGeo<IDX>& G) {
+  if constexpr (X < NSLOT) {
+    if (xr == (uint32_t)X) {
+      pgrp_x<X, MODE, IDX>(a, b, w, G);
+      __asm__ volatile("; pgrp arm %0" ::"n"(X));
+      return;
+    }
+    pgrp_switch<MODE, IDX, X + 1>(xr, a, b, w, G);
   }
 }
 
@@ -992,6 +1062,15 @@ __device__ __forceinline__ void jprot(float2 (&a)[NSLOT], float2 (&b)[NSLOT], cw
   const uint32_t xzpar = w[6] & 1u;
   prot_apply1<XR, Q, IDX>(a, G, z, c, sn, xzpar);
   if constexpr (MODE == MODE_DUAL) prot_apply1<XR, Q, IDX>(b, G, z, c, sn, xzpar);
+}
+
+template <int XR, int ACT, int MODE, typename IDX>
+__device__ __forceinline__ void jpgrp(float2 (&a)[NSLOT], float2 (&b)[NSLOT], cw_t w,
+                                      const PhaseGeo<IDX>& G) {
+  const IDX z0 = ld_idx<IDX>(w + 2);
+  const uint32_t kpar = w[4];
+  pgrp_apply1<XR, ACT, IDX>(a, G, z0, kpar, 0u, w + PG_HDR);
+  if constexpr (MODE == MODE_DUAL) pgrp_apply1<XR, ACT, IDX>(b, G, z0, kpar, 0u, w + PG_HDR);
 }
 
 template <int NWAVE, int NRED>

@@ -231,6 +231,34 @@ void run_steps(const uint32_t* w, uint32_t nsteps, cf* a, cf* b, bool dual, uint
         }
         break;
       }
+      case OP_PGRP: {
+        const uint32_t xr = w[1], kpar = w[4], act = w[5];
+        const uint64_t z0 = w64(w + 2);
+        const uint32_t lb = xr & (~xr + 1u), hi = xr & ~lb;
+        for (int z = 0; z < narr; ++z) {
+          cf* x = arrs[z];
+          for (int s = 0; s < NSLOT; ++s) {
+            if (s & lb) continue;
+            uint32_t cls = 0, k = 0;
+            for (int bb = 0; bb < RB; ++bb)
+              if ((hi >> bb) & 1u) {
+                if ((s >> bb) & 1) cls |= 1u << k;
+                ++k;
+              }
+            if (!((act >> cls) & 1u)) continue;
+            const uint32_t* e = w + PG_HDR + PG_ENT * cls;
+            const int s1 = s ^ (int)xr;
+            const uint32_t sgi = par64((gb ^ gs(s)) & z0), sgj = sgi ^ kpar;
+            const float cm1 = fw(e);
+            const cf ui = cf(fw(e + 1), fw(e + 2)) * (sgi ? -1.f : 1.f);
+            const cf uj = cf(fw(e + 3), fw(e + 4)) * (sgj ? -1.f : 1.f);
+            const cf x0 = x[s], x1 = x[s1];
+            x[s] = x0 + (cm1 * x0 + uj * x1);
+            x[s1] = x1 + (cm1 * x1 + ui * x0);
+          }
+        }
+        break;
+      }
       case OP_RED: {
         const uint32_t xr = w[1], kind = w[2], nterms = w[3];
         const uint32_t lb = xr & (~xr + 1u);
@@ -360,6 +388,17 @@ void apply_single_host(const Gate& g, int n, cf* psi) {
       for (uint64_t i = 0; i < dim; ++i) {
         const int s = 2 * (int)((i >> g.t0) & 1u) + (int)((i >> g.t1) & 1u);
         psi[i] *= cf(g.m[s]);
+      }
+      break;
+    case GK_PGRP:
+      for (int r = 0; r < g.nmem; ++r) {
+        Gate m = g;
+        m.kind = GK_PROT;
+        m.z = g.mz[r];
+        m.k = __builtin_popcountll(g.x & g.mz[r]) & 3;
+        m.c = std::cos(0.5 * g.ma[r]);
+        m.s = std::sin(0.5 * g.ma[r]);
+        apply_single_host(m, n, psi);
       }
       break;
     case GK_PROT: {

@@ -482,7 +482,22 @@ int plan_build_adjoint(tqsv_plan* p, hipStream_t st) {
   std::vector<Gate> r;
   for (int i = (int)p->gates.size() - 1; i >= 0; --i) {
     const Gate& g = p->gates[i];
-    if (g.param >= 0) {
+    if (g.kind == GK_PGRP) {
+      // every member commutes with the whole group, so all their reductions
+      // are taken at one point, before the group is undone
+      for (int mi = 0; mi < g.nmem; ++mi) {
+        if (g.mp[mi] < 0) continue;
+        Gate red;
+        red.kind = GK_RED;
+        red.x = g.x;
+        red.z = g.mz[mi];
+        red.k = popc(g.x & g.mz[mi]) & 3;
+        red.param = g.mp[mi];
+        red.gcoef = g.mg[mi];
+        classify_gate(red);
+        r.push_back(red);
+      }
+    } else if (g.param >= 0) {
       Gate red;
       red.kind = GK_RED;
       red.x = g.gx;

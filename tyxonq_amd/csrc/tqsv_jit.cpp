@@ -173,6 +173,9 @@ std::string gen_source(int tb, const uint32_t* pw, int n) {
           o << "      jprot<" << xr << ", " << q << ", MODE>(a, b, " << P << ", G);\n";
           break;
         }
+        case OP_PGRP:
+          o << "      jpgrp<" << w[1] << ", " << w[5] << ", MODE>(a, b, " << P << ", G);\n";
+          break;
         case OP_RED: {
           const uint32_t xr = w[1];
           if (racc) {

@@ -81,6 +81,9 @@ __device__ __forceinline__ void run_phases(cw_t prog, float2* tA, float2* tB, do
           case OP_PROT:
             if constexpr (!LEAN) prot_switch<MODE, IDX>(w[1], a, b, w, G);
             break;
+          case OP_PGRP:
+            if constexpr (!LEAN) pgrp_switch<MODE, IDX>(w[1], a, b, w, G);
+            break;
           case OP_RED:
             if constexpr (DUAL) red_switch<MODE, IDX>(w[1], a, b, w, G, acc, nred);
             break;
@@ -107,7 +110,7 @@ __device__ __forceinline__ void run_phases(cw_t prog, float2* tA, float2* tB, do
   }
 }
 
-// LEAN variants leave out the dense 4x4 (OP_U2) and Pauli-rotation (OP_PROT)
+// LEAN variants leave out the dense 4x4 (OP_U2) and Pauli-rotation (OP_PROT, OP_PGRP)
 // arms, which are most of the kernel's code: without them the kernel is about
 // a fifth of the size and every branch stays within the short-branch range.
 // The planner marks passes that contain neither step as lean.

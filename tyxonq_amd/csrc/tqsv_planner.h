@@ -22,7 +22,10 @@ enum GateKind : int {
   GK_DIAG2 = 3,  // diagonal d[2*bit(t0)+bit(t1)] (t0 == t1 for one bit)
   GK_PROT = 4,   // exp(-i theta/2 P)
   GK_RED = 5,    // cross reduction <lambda| P |psi> (adjoint sweeps)
+  GK_PGRP = 6,   // commuting Pauli rotations sharing one flip mask (members below)
 };
+
+constexpr int kMaxGroup = 16;  // members of a GK_PGRP gate
 
 struct Gate {
   int kind = 0;
@@ -48,7 +51,26 @@ struct Gate {
   // not depend on earlier parameters)
   int rot_form = -1;
   int src = -1;
+  // GK_PGRP: prod_r exp(-i ma[r]/2 P_r), P_r = (x, mz[r]); the members commute
+  // and their z masks differ only inside x.  mp[r], mg[r]: the member's
+  // parameter (-1: none) and generator coefficient (angle = mg[r] * param +
+  // const) for the adjoint.  `param` is the largest member parameter.
+  int nmem = 0;
+  uint64_t mz[kMaxGroup] = {};
+  double ma[kMaxGroup] = {};
+  double mg[kMaxGroup] = {};
+  int mp[kMaxGroup] = {};
 };
+
+// Class constants of a GK_PGRP gate for its pairs (i, i ^ x), one class per
+// pattern of the x bits above the lowest one (compressed, ascending bits).
+struct PgrpClass {
+  bool active = false;  // false: the identity on these pairs, whatever the angles
+  double cm1 = 0.0;     // cos(w/2) - 1
+  cd ui{0, 0}, uj{0, 0};
+};
+// Returns the number of classes (2^(|x|-1)).
+int pgrp_classes(const Gate& g, PgrpClass* out);
 
 // Convert ABI ops into gates (validates).  Returns 0 or an error code.
 int gates_from_ops(int n, const tqsv_op* ops, int nops, std::vector<Gate>& out);

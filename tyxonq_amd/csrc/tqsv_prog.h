@@ -84,9 +84,10 @@ enum : uint32_t {
   // 2-bit diagonal on arbitrary index bits: amp *= d[2*bit(ga)+bit(gb)]
   // [1]=ga [2]=gb [3]=pad [4..12)=d (4 complex)
   OP_D2 = 8,
-  // Pauli rotation exp(-i theta/2 P), flip bits inside the register slots.
-  // [1]=xr (slot mask) [2..4)=z (lo,hi) [4]=cos(theta/2) [5]=sin(theta/2)
-  // [6]=k with omega = i^k, k=|x&z| mod 4
+  // Pauli rotation exp(-i theta/2 P) up to a real factor, flip bits inside the
+  // register slots: a -> C a - i S P a.  [1]=xr (slot mask) [2..4)=z (lo,hi)
+  // [4]=C [5]=S (one of them 1, the other tan or cot of theta/2; the factor
+  // joins the deferred scale) [6]=k with omega = i^k, k=|x&z| mod 4
   OP_PROT = 9,
   // reduction group.  [1]=xr [2]=kind [3]=nterms, then nterms entries
   // {slot, zt lo, zt hi, smask, part} = 5 words.
@@ -105,6 +106,16 @@ enum : uint32_t {
   // the slots whose register bits pass the control (all slots unless creg is
   // a slot bit)
   OP_CXP = 12,
+  // commuting Pauli rotations sharing the flip mask x (register slots),
+  // prod_r exp(-i a_r/2 P_r) with P_r = (x, z0 ^ d_r), d_r inside x.  On the
+  // pair (i, j = i ^ x), i the member with the lowest bit of x clear, it is
+  //   new_i = a_i + cm1 a_i + sg(j) u_j a_j,  new_j = a_j + cm1 a_j + sg(i) u_i a_i
+  // with sg(k) = (-1)^{|k & z0|} and the class constants of the pair's
+  // pattern on x (cm1 = cos - 1, u = -i sin/w times the generator entry).
+  // [1]=xr [2..4)=z0 (lo,hi) [4]=|x & z0| mod 2 [5]=active class mask, then
+  // per class (pattern of the x bits above the lowest one, compressed)
+  // {cm1, u_i re, u_i im, u_j re, u_j im}; inactive classes are the identity.
+  OP_PGRP = 13,
 };
 
 constexpr uint32_t CX_THREAD = RB;
@@ -127,6 +138,8 @@ constexpr int THR_ENT = 4;
 constexpr int TAB_OFF = 4;
 constexpr int RED_HDR = 4;
 constexpr int RED_ENT = 5;
+constexpr int PG_HDR = 6;
+constexpr int PG_ENT = 5;
 
 }  // namespace tqsv
 
