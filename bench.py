@@ -46,7 +46,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="brickwork",
-                    choices=["brickwork", "adjoint", "hea", "ucc", "serve", "sample"])
+                    choices=["brickwork", "adjoint", "hea", "ucc", "serve", "sample", "mfma"])
     ap.add_argument("--n", type=int, default=None)
     ap.add_argument("--layers", type=int, default=None)
     ap.add_argument("--terms", type=int, default=500)
@@ -662,6 +662,105 @@ def serve_at(rate: float, requests: int, slots: int, seed: int) -> dict:
             "p99_vs_plain": round(srv["p99_ms"] / plain["p99_ms"], 3)}
 
 
+# ---- dense 5-qubit blocks: matrix cores against the VALU paths ---------------------
+def run_mfma(args) -> dict:
+    """Dense 5-qubit blocks on the 30-qubit state.  A 32x32 complex unitary
+    (any number of gates on 5 qubits fuses into one; here a random one) is
+    applied to the 5 lowest index bits of all 2^25 blocks (a) as one complex
+    GEMM on the matrix cores (torch.matmul on complex64 -> the vendor BLAS
+    cgemm, FP32 MFMA), (b) by the VALU k-qubit kernel of this library
+    (tqsv_apply_kq, one output amplitude per thread), and both are set
+    against (c) the fused tile pass of the 30-qubit brickwork plan, which
+    applies about 28 single-qubit gates (RX.RZ) plus CZs per amplitude per
+    pass.  A dense block costs 32 complex MACs (256 flops) per amplitude
+    whatever it fuses, and covers only the 5 qubits of its block."""
+    import torch
+
+    from tyxonq_amd import _native as nat
+    from tyxonq_amd import bench_workloads as W
+    from tyxonq_amd.engine import lower
+
+    n = args.n or 30
+    dim = 1 << n
+    dev = nat.device()
+    rng = np.random.default_rng(args.seed)
+    A = rng.normal(size=(32, 32)) + 1j * rng.normal(size=(32, 32))
+    U = np.linalg.qr(A)[0]
+    g = torch.Generator(device=dev).manual_seed(args.seed)
+    psi = torch.complex(torch.randn(dim, device=dev, generator=g),
+                        torch.randn(dim, device=dev, generator=g)).to(torch.complex64)
+    psi /= torch.linalg.vector_norm(psi)
+    UT = torch.tensor(U.T.copy(), dtype=torch.complex64, device=dev)
+    out = torch.empty_like(psi)
+    ev0, ev1 = _events()
+    reps = max(args.steps, 3)
+
+    def timed(fn):
+        for _ in range(max(args.warmup, 1)):
+            fn()
+        _sync()
+        ev0.record()
+        for _ in range(reps):
+            fn()
+        ev1.record()
+        ev1.synchronize()
+        return ev0.elapsed_time(ev1) / reps
+
+    ms_mfma = timed(lambda: torch.matmul(psi.view(dim // 32, 32), UT, out=out.view(dim // 32, 32)))
+    got_mfma = out[: 1 << 20].clone()
+    res = {}
+
+    def valu():
+        res["o"] = nat.apply_kq(psi, [4, 3, 2, 1, 0], U)
+
+    ms_valu = timed(valu)
+    diff = float((res["o"][: 1 << 20] - got_mfma).abs().max().item())
+    del res
+    torch.cuda.empty_cache()
+    circ = W.brickwork_circuit(n, 20, args.seed)
+    L, _, _ = lower(circ)
+    plan = nat.Plan(n, L.segments[0][1], max_cost=260)
+    passes = int(plan.info()["passes"])
+    ms_plan = timed(lambda: plan.run(psi)) if passes else 0.0
+    nbytes = 16 * dim
+    flops = 256 * dim
+
+    def line(ms):
+        return {"ms": round(ms, 4), "GBps": round(nbytes / (ms * 1e-3) / 1e9, 1),
+                "TFLOPs": round(flops / (ms * 1e-3) / 1e12, 2)}
+
+    a, b = line(ms_mfma), line(ms_valu)
+    per_pass = ms_plan / max(passes, 1)
+    return {
+        "metric": METRIC,
+        "value": a["GBps"],
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": reps,
+        "warmup": args.warmup,
+        "ms_per_step": a["ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64",
+        "data": "synthetic (seeded random state and random 32x32 unitary)",
+        "config": {"workload": f"dense 32x32 complex block on the 5 low bits of a {n}-qubit state",
+                   "qubits": n, "parallelism": "1 GPU"},
+        "mfma_cgemm": a,
+        "valu_kq_kernel": b,
+        "max_abs_diff_mfma_vs_valu": diff,
+        "fused_pass": {"ms": round(per_pass, 4),
+                       "GBps": round(nbytes / (per_pass * 1e-3) / 1e9, 1),
+                       "single_qubit_gates_per_pass": round(2 * n * 20 / max(passes, 1), 1),
+                       "passes": passes, "plan_ms": round(ms_plan, 3)},
+        "roofline": {"bound": "hbm", "achieved": a["GBps"], "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": round(a["GBps"] / HBM_PEAK_GBPS, 4), "traffic": None,
+                     "kernel": "vendor cgemm (FP32 MFMA) applying the dense block",
+                     "bytes_per_launch": nbytes, "avg_launch_ms": a["ms"]},
+        "cpu_baseline": None,
+    }
+
+
 # ---- shot sampling (counts path of engine.run) ------------------------------------
 def run_sample(args) -> dict:
     """Counts of `shots` draws from a 30-qubit state (8 GiB, random amplitudes)
@@ -752,6 +851,8 @@ def main():
         out = run_serve(args)
     elif args.workload == "sample":
         out = run_sample(args)
+    elif args.workload == "mfma":
+        out = run_mfma(args)
     else:
         out = run_brickwork(args)
         if not args.no_extras:

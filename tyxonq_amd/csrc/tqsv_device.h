@@ -115,6 +115,25 @@ __device__ __forceinline__ double wave_sum_d(double v) {
   v += shfl_xor_d(v, 32);
   return v;
 }
+// DPP on the rows selected by ROWMASK (other lanes get 0)
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dpp_rows_d(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROWMASK, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROWMASK, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+// The wave total in lane 63 only, without the LDS round trips of the two
+// cross-row shuffles: after the in-row butterfly, row_bcast:15 adds row 0 to
+// row 1 and row 2 to row 3, row_bcast:31 adds rows 0+1 to rows 2 and 3.
+__device__ __forceinline__ double wave_sum_last_d(double v) {
+  v += dpp_d<0xB1>(v);
+  v += dpp_d<0x4E>(v);
+  v += dpp_d<0x141>(v);
+  v += dpp_d<0x140>(v);
+  v += dpp_rows_d<0x142, 0xa>(v);
+  v += dpp_rows_d<0x143, 0xc>(v);
+  return v;
+}
 
 // ---- OP_U1 --------------------------------------------------------------
 struct M2f {
@@ -673,10 +692,10 @@ __device__ __forceinline__ void red_cross(const float2 (&a)[NSLOT], const float2
     vr = -vr;
     vi = -vi;
   }
-  const double dr = wave_sum_d((double)vr);
-  const double di = wave_sum_d((double)vi);
+  const double dr = wave_sum_last_d((double)vr);
+  const double di = wave_sum_last_d((double)vi);
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-  if (lane == 0) {
+  if (lane == 63) {
     acc[wave * nred + e[0]] += dr;
     acc[wave * nred + e[RED_ENT]] += di;
   }
@@ -725,8 +744,8 @@ __device__ __forceinline__ void red_x(const float2 (&a)[NSLOT], const float2 (&b
       }
     }
     v = parity((IDX)(G.gb & zt)) ? -v : v;
-    const double d = wave_sum_d((double)v);
-    if (lane == 0) acc[wave * nred + slot] += d;
+    const double d = wave_sum_last_d((double)v);
+    if (lane == 63) acc[wave * nred + slot] += d;
   }
 }
 
