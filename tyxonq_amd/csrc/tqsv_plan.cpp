@@ -1054,10 +1054,6 @@ int effective_tile_bits(int n, int requested, bool dual) {
   if (T < 11) T = 11;
   if (T > 14) T = 14;
   if (dual && T > 13) T = 13;
-  if (dual) {
-    // experiment switch: tile bits of dual (adjoint) passes
-    if (const char* e = std::getenv("TQSV_DUAL_T"); e && e[0]) T = std::min(T, std::max(11, std::atoi(e)));
-  }
   if (n > 32) T = 13;  // 64-bit index kernels exist for 2^13-amplitude tiles only
   if (n < T) T = n;
   return T;
