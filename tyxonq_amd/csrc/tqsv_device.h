@@ -749,6 +749,49 @@ __device__ __forceinline__ void red_x(const float2 (&a)[NSLOT], const float2 (&b
   }
 }
 
+// Group reduction of a dual pass (kind RED_GROUP): Im <lambda| G |psi> for
+// the generator G = sum_r g_r P_r of a rotation group with one shared
+// parameter, in one sweep over the thread's pairs and one wave sum, instead
+// of one reduction (two wave sums) per Pauli string.  ACT >= 0: active class
+// mask fixed at compile time.
+template <int XR, int ACT, typename IDX>
+__device__ __forceinline__ void red_group(const float2 (&a)[NSLOT], const float2 (&b)[NSLOT],
+                                          cw_t w, const PhaseGeo<IDX>& G, double* acc,
+                                          uint32_t nred) {
+  constexpr int LB = XR & (-XR);
+  constexpr uint32_t HI = (uint32_t)XR & ~(uint32_t)LB;
+  cw_t e = w + RED_HDR;
+  const IDX z0 = ld_idx<IDX>(e + 1);
+  cw_t t = e + RED_ENT;
+  const uint32_t kpar = t[0], act = t[1];
+  cw_t tab = t + 2;
+  const uint32_t tp = parity((IDX)(G.gb & z0));
+  float vi = 0.f;
+#pragma unroll
+  for (int s = 0; s < NSLOT; ++s) {
+    if (s & LB) continue;
+    const uint32_t cls = pext_slot<HI>((uint32_t)s);
+    if (ACT >= 0 ? !(((uint32_t)ACT >> cls) & 1u) : !((act >> cls) & 1u)) continue;
+    const int s1 = s ^ XR;
+    cw_t c = tab + RGRP_ENT * cls;
+    const uint32_t sgi = tp ^ parity((IDX)(gs_of(G, s) & z0));
+    const uint32_t sgj = sgi ^ kpar;
+    // Im(conj(l) w a) = l.x (w.r a.y + w.i a.x) - l.y (w.r a.x - w.i a.y)
+    const float wjr = sgj ? -fl(c + 2) : fl(c + 2), wji = sgj ? -fl(c + 3) : fl(c + 3);
+    const float wir = sgi ? -fl(c) : fl(c), wii = sgi ? -fl(c + 1) : fl(c + 1);
+    const float2 li = b[s], lj = b[s1], ai = a[s], aj = a[s1];
+    const float qjr = wjr * aj.x - wji * aj.y, qji = wjr * aj.y + wji * aj.x;
+    const float qir = wir * ai.x - wii * ai.y, qii = wir * ai.y + wii * ai.x;
+    vi = fmaf(li.x, qji, vi);
+    vi = fmaf(-li.y, qjr, vi);
+    vi = fmaf(lj.x, qii, vi);
+    vi = fmaf(-lj.y, qir, vi);
+  }
+  const double d = wave_sum_last_d((double)vi);
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  if (lane == 63) acc[wave * nred + e[0]] += d;
+}
+
 template <int XR, int MODE, typename IDX>
 __device__ __forceinline__ void red_dispatch_kind(const float2 (&a)[NSLOT],
                                                   const float2 (&b)[NSLOT], cw_t w,
@@ -757,6 +800,8 @@ __device__ __forceinline__ void red_dispatch_kind(const float2 (&a)[NSLOT],
   const uint32_t kind = w[2];
   if constexpr (MODE == MODE_DUAL) {
     if (kind == RED_CROSS) red_cross<XR, IDX>(a, b, w, G, acc, nred);
+    if constexpr (XR != 0)
+      if (kind == RED_GROUP) red_group<XR, -1, IDX>(a, b, w, G, acc, nred);
   } else {
     if constexpr (XR == 0) {
       red_x<0, RED_DIAG, MODE, IDX>(a, b, w, G, acc, nred);

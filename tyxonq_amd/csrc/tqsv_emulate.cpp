@@ -262,6 +262,32 @@ void run_steps(const uint32_t* w, uint32_t nsteps, cf* a, cf* b, bool dual, uint
       case OP_RED: {
         const uint32_t xr = w[1], kind = w[2], nterms = w[3];
         const uint32_t lb = xr & (~xr + 1u);
+        if (kind == RED_GROUP) {
+          const uint32_t* e = w + RED_HDR;
+          const uint64_t z0 = w64(e + 1);
+          const uint32_t kpar = e[RED_ENT], act = e[RED_ENT + 1];
+          const uint32_t* tab = e + RED_ENT + 2;
+          const uint32_t hi = xr & ~lb;
+          double acc = 0.0;
+          for (int s = 0; s < NSLOT; ++s) {
+            if (s & lb) continue;
+            uint32_t cls = 0, k = 0;
+            for (int bb = 0; bb < RB; ++bb)
+              if ((hi >> bb) & 1u) {
+                if ((s >> bb) & 1) cls |= 1u << k;
+                ++k;
+              }
+            if (!((act >> cls) & 1u)) continue;
+            const uint32_t* c = tab + RGRP_ENT * cls;
+            const int s1 = s ^ (int)xr;
+            const uint32_t sgi = par64((gb ^ gs(s)) & z0), sgj = sgi ^ kpar;
+            const cf wi = cf(fw(c), fw(c + 1)) * (sgi ? -1.f : 1.f);
+            const cf wj = cf(fw(c + 2), fw(c + 3)) * (sgj ? -1.f : 1.f);
+            acc += (std::conj(b[s]) * wj * a[s1] + std::conj(b[s1]) * wi * a[s]).imag();
+          }
+          if (red) (*red)[e[0]] += acc;
+          break;
+        }
         const uint32_t* e = w + RED_HDR;
         for (uint32_t t = 0; t < nterms; ++t, e += RED_ENT) {
           double acc = 0.0;

@@ -482,7 +482,19 @@ int plan_build_adjoint(tqsv_plan* p, hipStream_t st) {
   std::vector<Gate> r;
   for (int i = (int)p->gates.size() - 1; i >= 0; --i) {
     const Gate& g = p->gates[i];
-    if (g.kind == GK_PGRP) {
+    bool shared = g.kind == GK_PGRP && g.nmem > 0 && g.mp[0] >= 0;
+    for (int mi = 1; shared && mi < g.nmem; ++mi) shared = g.mp[mi] == g.mp[0];
+    if (shared) {
+      // one parameter for the whole group: dE/dtheta = Im <lambda| G |psi>
+      // with G = sum_r mg[r] P_r, one reduction over the group's pairs
+      Gate red = g;
+      red.kind = GK_RED;
+      red.k = 0;
+      red.param = g.mp[0];
+      red.gcoef = 1.0;
+      classify_gate(red);
+      r.push_back(red);
+    } else if (g.kind == GK_PGRP) {
       // every member commutes with the whole group, so all their reductions
       // are taken at one point, before the group is undone
       for (int mi = 0; mi < g.nmem; ++mi) {
@@ -558,6 +570,10 @@ static void collect_grad(const tqsv_plan* p, const double* hred, int nparams, do
   for (int i = 0; i < nparams; ++i) grad[i] = 0.0;
   for (const Gate& g : p->rgates) {
     if (g.kind != GK_RED) continue;
+    if (g.nmem > 0) {  // group reduction: one slot holding Im <lambda|G|psi>
+      grad[g.param] += scale * g.gcoef * g.red_scale * hred[g.red_slot];
+      continue;
+    }
     const double re = hred[g.red_slot], im = hred[g.red_slot + 1];
     double v;
     switch (g.k) {

@@ -192,7 +192,10 @@ std::string gen_source(int tb, const uint32_t* pw, int n) {
             o << "      }\n";
             break;
           }
-          if (mode == MODE_DUAL)
+          if (mode == MODE_DUAL && w[2] == RED_GROUP)
+            o << "      red_group<" << xr << ", " << w[RED_HDR + RED_ENT + 1] << ", IDX>(a, b, " << P
+              << ", G, acc, " << nred << ");\n";
+          else if (mode == MODE_DUAL)
             o << "      red_cross<" << xr << ", IDX>(a, b, " << P << ", G, acc, " << nred << ");\n";
           else
             o << "      red_x<" << xr << ", " << (xr ? RED_PAIR : RED_DIAG) << ", MODE, IDX>(a, b, "

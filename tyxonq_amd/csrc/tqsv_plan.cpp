@@ -68,7 +68,7 @@ int gate_cost(const Gate& g) {
     case GK_MAT2: return g.cmask ? 20 : 16;
     case GK_DIAG2: return 1;
     case GK_PROT: return 8;
-    case GK_RED: return 10;
+    case GK_RED: return g.nmem > 0 ? 6 : 10;
     case GK_PGRP: {
       PgrpClass cls[kMaxGroup];
       const int nc = pgrp_classes(g, cls);
@@ -174,6 +174,13 @@ void classify(Gate& g) {
     case GK_DIAG2: g.db = bitm(g.t0) | bitm(g.t1); break;
     case GK_PROT:
     case GK_RED:
+      if (g.kind == GK_RED && g.nmem > 0) {
+        uint64_t zu = g.z;
+        for (int r = 0; r < g.nmem; ++r) zu |= g.mz[r];
+        g.nb = g.x;
+        g.db = zu & ~g.x;
+        break;
+      }
       g.nb = g.x;
       g.db = g.z & ~g.x;
       break;
@@ -804,6 +811,50 @@ void emit_phase(std::vector<Gate>& gates, const std::vector<int>& sel,
         // the terms on the flip bits are applied first; the rest cancels
         // except |exp(c0)|^2, which the host applies to the result.
         extract_bits(g.x);
+        if (g.nmem > 0) {
+          // group generator G = sum_r mg[r] P_r: one slot, Im <lambda|G|psi>
+          Gate gen = g;
+          for (int r = 0; r < g.nmem; ++r) gen.ma[r] = g.mg[r];
+          PgrpClass cls[kMaxGroup];
+          const int nc = pgrp_classes(gen, cls);
+          int xb[64];
+          int m = 0;
+          for (uint64_t v = g.x; v; v &= v - 1) xb[m++] = ctz(v);
+          static const cd ipow[4] = {cd(1, 0), cd(0, 1), cd(-1, 0), cd(0, -1)};
+          auto Vg = [&](uint64_t pat) {
+            cd acc(0, 0);
+            for (int r = 0; r < g.nmem; ++r) {
+              const uint64_t d = (g.mz[r] ^ g.z) & g.x;
+              acc += g.mg[r] * ((popc(pat & d) & 1) ? -1.0 : 1.0) * ipow[popc(g.x & g.mz[r]) & 3];
+            }
+            return acc;
+          };
+          uint32_t act = 0;
+          for (int c = 0; c < nc; ++c)
+            if (cls[c].active) act |= 1u << c;
+          const size_t at = e.begin(OP_RED);
+          e.u(slotmask(g.x));
+          e.u(RED_GROUP);
+          e.u(1);
+          e.u(nred);
+          e.u64(g.z);
+          e.u(0);
+          e.u(1);
+          e.u((uint32_t)(popc(g.x & g.z) & 1));
+          e.u(act);
+          for (int c = 0; c < nc; ++c) {
+            uint64_t pi = 0;
+            for (int b = 1; b < m; ++b)
+              if ((c >> (b - 1)) & 1) pi |= bitm(xb[b]);
+            e.c(Vg(pi));
+            e.c(Vg(pi ^ g.x));
+          }
+          e.end(at);
+          g.red_slot = (int)(red_base + nred);
+          g.red_scale = std::exp(2.0 * P.c0.real());
+          nred += 1;
+          break;
+        }
         const uint32_t xr = slotmask(g.x);
         const uint32_t zs = slotmask(g.z);
         const uint32_t sm = smask_of(zs);
@@ -1223,7 +1274,7 @@ int select_passes(std::vector<Gate>& gates, const SchedOpts& so, std::vector<Pas
       }
       chosen.push_back(gi);
       budget -= g.cost;
-      if (g.kind == GK_RED) nred_pass += 2;
+      if (g.kind == GK_RED) nred_pass += g.nmem > 0 ? 1 : 2;
     }
     if (chosen.empty()) return fail(TQSV_EINVAL, "planner made no progress (internal error)");
     // 3) phases

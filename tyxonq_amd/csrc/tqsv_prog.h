@@ -95,6 +95,12 @@ enum : uint32_t {
   //  kind 1: all s: v = |a[s]|^2 (xr = 0)
   //  kind 2: all s: v = conj(b[s]) a[s^xr]  (b = lambda, dual passes)
   // term value += sum_s (-1)^{smask bit s} (-1)^{|gb & zt|} part(v)
+  //  kind 3 (dual passes, one entry; zt = z0 of a rotation group): the
+  //    imaginary part of <lambda| G |psi> for the group generator G =
+  //    sum_r g_r P_r, summed over the pairs (i, j = i ^ x) of the active
+  //    classes: conj(b_i) sg(j) V_j a_j + conj(b_j) sg(i) V_i a_i, with
+  //    sg(k) = (-1)^{|k & z0|}.  After the entry: [kpar] [active class mask]
+  //    then per class {V_i re, V_i im, V_j re, V_j im}.
   OP_RED = 10,
   // real scale: a[s] *= f for all s.  [1]=f (float)
   OP_SCL = 11,
@@ -130,6 +136,8 @@ constexpr uint32_t U1_RXLIKE = 2;  // real diagonal, imaginary off-diagonal
 constexpr uint32_t RED_PAIR = 0;
 constexpr uint32_t RED_DIAG = 1;
 constexpr uint32_t RED_CROSS = 2;
+constexpr uint32_t RED_GROUP = 3;
+constexpr int RGRP_ENT = 4;
 
 constexpr int OPH_HDR = 6;
 constexpr int OPH_ENT = 3;
