@@ -621,11 +621,15 @@ __device__ __forceinline__ constexpr uint32_t pext_slot(uint32_t s) {
 // A group of commuting Pauli rotations on the pairs (s, s ^ XR): per pair
 // class the two-level rotation new_i = a_i + (cm1 a_i + sg(j) u_j a_j),
 // new_j = a_j + (cm1 a_j + sg(i) u_i a_i), the small bracket first so that
-// each amplitude is rounded once at its own size.  ACT >= 0: the active class
-// mask fixed at compile time (generated kernels); ACT < 0: read at run time.
-template <int XR, int ACT, typename IDX>
+// each amplitude is rounded once at its own size.  sg(i) = (-1)^{tp + bit s of
+// the slot sign mask}, tp the parity of the thread's bits & z0.
+// CT (generated kernels): the active class mask ACT, the slot sign mask SM
+// and |x & z0| mod 2 KP are compile-time constants, so the slot signs become
+// FMA source modifiers and only tp is applied at run time (once per class).
+// Otherwise they are the uniform runtime values act, sm, kpar.
+template <int XR, bool CT, uint32_t ACT, uint32_t SM, uint32_t KP, typename IDX>
 __device__ __forceinline__ void pgrp_apply1(float2 (&a)[NSLOT], const PhaseGeo<IDX>& G, IDX z0,
-                                            uint32_t kpar, uint32_t act, cw_t tab) {
+                                            uint32_t kpar, uint32_t act, uint32_t sm, cw_t tab) {
   constexpr int LB = XR & (-XR);
   constexpr uint32_t HI = (uint32_t)XR & ~(uint32_t)LB;
   const uint32_t tp = parity((IDX)(G.gb & z0));
@@ -633,14 +637,29 @@ __device__ __forceinline__ void pgrp_apply1(float2 (&a)[NSLOT], const PhaseGeo<I
   for (int s = 0; s < NSLOT; ++s) {
     if (s & LB) continue;
     const uint32_t cls = pext_slot<HI>((uint32_t)s);
-    if (ACT >= 0 ? !(((uint32_t)ACT >> cls) & 1u) : !((act >> cls) & 1u)) continue;
+    if (CT ? !((ACT >> cls) & 1u) : !((act >> cls) & 1u)) continue;
     const int s1 = s ^ XR;
     const cw_t e = tab + PG_ENT * cls;
     const float cm1 = fl(e);
-    const uint32_t sgi = tp ^ parity((IDX)(gs_of(G, s) & z0));
-    const uint32_t sgj = sgi ^ kpar;
-    const float uir = sgi ? -fl(e + 1) : fl(e + 1), uii = sgi ? -fl(e + 2) : fl(e + 2);
-    const float ujr = sgj ? -fl(e + 3) : fl(e + 3), uji = sgj ? -fl(e + 4) : fl(e + 4);
+    float uir, uii, ujr, uji;
+    if constexpr (CT) {
+      // the thread sign once per class (common subexpression across the
+      // class's pairs), the slot signs at compile time
+      const float tir = tp ? -fl(e + 1) : fl(e + 1), tii = tp ? -fl(e + 2) : fl(e + 2);
+      const float tjr = tp ? -fl(e + 3) : fl(e + 3), tji = tp ? -fl(e + 4) : fl(e + 4);
+      const bool bi = (SM >> s) & 1u, bj = bi != ((KP & 1u) != 0u);
+      uir = bi ? -tir : tir;
+      uii = bi ? -tii : tii;
+      ujr = bj ? -tjr : tjr;
+      uji = bj ? -tji : tji;
+    } else {
+      const uint32_t sgi = tp ^ ((sm >> s) & 1u);
+      const uint32_t sgj = sgi ^ kpar;
+      uir = sgi ? -fl(e + 1) : fl(e + 1);
+      uii = sgi ? -fl(e + 2) : fl(e + 2);
+      ujr = sgj ? -fl(e + 3) : fl(e + 3);
+      uji = sgj ? -fl(e + 4) : fl(e + 4);
+    }
     const float2 x0 = a[s], x1 = a[s1];
     const float t0r = fmaf(-uji, x1.y, fmaf(cm1, x0.x, ujr * x1.x));
     const float t0i = fmaf(uji, x1.x, fmaf(cm1, x0.y, ujr * x1.y));
@@ -656,9 +675,10 @@ __device__ __forceinline__ void pgrp_x(float2 (&a)[NSLOT], float2 (&b)[NSLOT], c
                                        const PhaseGeo<IDX>& G) {
   if constexpr (XR != 0) {
     const IDX z0 = ld_idx<IDX>(w + 2);
-    const uint32_t kpar = w[4], act = w[5];
-    pgrp_apply1<XR, -1, IDX>(a, G, z0, kpar, act, w + PG_HDR);
-    if constexpr (MODE == MODE_DUAL) pgrp_apply1<XR, -1, IDX>(b, G, z0, kpar, act, w + PG_HDR);
+    const uint32_t kpar = w[4], act = w[5], sm = w[6];
+    pgrp_apply1<XR, false, 0u, 0u, 0u, IDX>(a, G, z0, kpar, act, sm, w + PG_HDR);
+    if constexpr (MODE == MODE_DUAL)
+      pgrp_apply1<XR, false, 0u, 0u, 0u, IDX>(b, G, z0, kpar, act, sm, w + PG_HDR);
   }
 }
 
@@ -670,35 +690,60 @@ __device__ __forceinline__ float sgn1(uint32_t mask, int s) {
 
 // Per term: FP32 sum over the thread's 32 slots, then FP64 over the wave's
 // lanes, the LDS accumulator of the wave and beyond.
-// Cross group of a dual pass: exactly two entries (real and imaginary part)
-// with the same z; both sums are formed in one sweep over the slots (FP32,
-// 32 products per thread) and reduced over the wave in FP64.
+// Cross reduction of a dual pass (one entry): one part (PART 0 real, 1
+// imaginary) of sum_s sg(s) conj(b[s]) a[s ^ XR], the part the gradient of
+// the rotation needs.  Two FP32 accumulators (even and odd slots) keep the
+// FMA chains short.  CT: the slot sign mask SM is a compile-time constant
+// (generated kernels; the signs become FMA source modifiers), else the
+// uniform runtime mask `sm`.
+template <int XR, int PART, bool CT, uint32_t SM>
+__device__ __forceinline__ float cross_part(const float2 (&a)[NSLOT], const float2 (&b)[NSLOT],
+                                            uint32_t sm) {
+  float v[2] = {0.f, 0.f};
+#pragma unroll
+  for (int s = 0; s < NSLOT; ++s) {
+    const float2 u = b[s], w = a[s ^ XR];  // conj(u) * w
+    float& acc = v[s & 1];
+    if constexpr (CT) {
+      const bool neg = (SM >> s) & 1u;
+      if constexpr (PART == 0) {
+        acc = fmaf(neg ? -u.x : u.x, w.x, acc);
+        acc = fmaf(neg ? -u.y : u.y, w.y, acc);
+      } else {
+        acc = fmaf(neg ? -u.x : u.x, w.y, acc);
+        acc = fmaf(neg ? u.y : -u.y, w.x, acc);
+      }
+    } else {
+      const float q = PART == 0 ? u.x * w.x + u.y * w.y : u.x * w.y - u.y * w.x;
+      acc = fmaf(sgn1(sm, s), q, acc);
+    }
+  }
+  return v[0] + v[1];
+}
+
 template <int XR, typename IDX>
 __device__ __forceinline__ void red_cross(const float2 (&a)[NSLOT], const float2 (&b)[NSLOT],
                                           cw_t w, const PhaseGeo<IDX>& G, double* acc,
                                           uint32_t nred) {
   cw_t e = w + RED_HDR;
   const IDX zt = ld_idx<IDX>(e + 1);
-  const uint32_t smask = e[3];
-  float vr = 0.f, vi = 0.f;
-#pragma unroll
-  for (int s = 0; s < NSLOT; ++s) {
-    const float2 u = b[s], v = a[s ^ XR];  // conj(u) * v
-    const float sg = sgn1(smask, s);
-    vr = fmaf(sg, u.x * v.x + u.y * v.y, vr);
-    vi = fmaf(sg, u.x * v.y - u.y * v.x, vi);
-  }
-  if (parity((IDX)(G.gb & zt))) {
-    vr = -vr;
-    vi = -vi;
-  }
-  const double dr = wave_sum_last_d((double)vr);
-  const double di = wave_sum_last_d((double)vi);
+  float v = e[4] ? cross_part<XR, 1, false, 0u>(a, b, e[3]) : cross_part<XR, 0, false, 0u>(a, b, e[3]);
+  if (parity((IDX)(G.gb & zt))) v = -v;
+  const double d = wave_sum_last_d((double)v);
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-  if (lane == 63) {
-    acc[wave * nred + e[0]] += dr;
-    acc[wave * nred + e[RED_ENT]] += di;
-  }
+  if (lane == 63) acc[wave * nred + e[0]] += d;
+}
+
+// the cross reduction of generated kernels: slot, z mask, sign mask and part
+// fixed at compile time
+template <int XR, int PART, uint32_t SM, uint32_t SLOT, uint32_t NRED, typename IDX>
+__device__ __forceinline__ void jred_cross(const float2 (&a)[NSLOT], const float2 (&b)[NSLOT],
+                                           const PhaseGeo<IDX>& G, IDX zt, double* acc) {
+  float v = cross_part<XR, PART, true, SM>(a, b, 0u);
+  if (parity((IDX)(G.gb & zt))) v = -v;
+  const double d = wave_sum_last_d((double)v);
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  if (lane == 63) acc[wave * NRED + SLOT] += d;
 }
 
 template <int XR, int KIND, int MODE, typename IDX>
@@ -752,33 +797,41 @@ __device__ __forceinline__ void red_x(const float2 (&a)[NSLOT], const float2 (&b
 // Group reduction of a dual pass (kind RED_GROUP): Im <lambda| G |psi> for
 // the generator G = sum_r g_r P_r of a rotation group with one shared
 // parameter, in one sweep over the thread's pairs and one wave sum, instead
-// of one reduction (two wave sums) per Pauli string.  ACT >= 0: active class
-// mask fixed at compile time.
-template <int XR, int ACT, typename IDX>
-__device__ __forceinline__ void red_group(const float2 (&a)[NSLOT], const float2 (&b)[NSLOT],
-                                          cw_t w, const PhaseGeo<IDX>& G, double* acc,
-                                          uint32_t nred) {
+// of one reduction (two wave sums) per Pauli string.  CT: active class mask,
+// slot sign mask and kpar fixed at compile time (as for pgrp_apply1).
+template <int XR, bool CT, uint32_t ACT, uint32_t SM, uint32_t KP, typename IDX>
+__device__ __forceinline__ float group_part(const float2 (&a)[NSLOT], const float2 (&b)[NSLOT],
+                                            const PhaseGeo<IDX>& G, IDX z0, uint32_t kpar,
+                                            uint32_t act, uint32_t sm, cw_t tab) {
   constexpr int LB = XR & (-XR);
   constexpr uint32_t HI = (uint32_t)XR & ~(uint32_t)LB;
-  cw_t e = w + RED_HDR;
-  const IDX z0 = ld_idx<IDX>(e + 1);
-  cw_t t = e + RED_ENT;
-  const uint32_t kpar = t[0], act = t[1];
-  cw_t tab = t + 2;
   const uint32_t tp = parity((IDX)(G.gb & z0));
   float vi = 0.f;
 #pragma unroll
   for (int s = 0; s < NSLOT; ++s) {
     if (s & LB) continue;
     const uint32_t cls = pext_slot<HI>((uint32_t)s);
-    if (ACT >= 0 ? !(((uint32_t)ACT >> cls) & 1u) : !((act >> cls) & 1u)) continue;
+    if (CT ? !((ACT >> cls) & 1u) : !((act >> cls) & 1u)) continue;
     const int s1 = s ^ XR;
     cw_t c = tab + RGRP_ENT * cls;
-    const uint32_t sgi = tp ^ parity((IDX)(gs_of(G, s) & z0));
-    const uint32_t sgj = sgi ^ kpar;
+    float wir, wii, wjr, wji;
+    if constexpr (CT) {
+      const float tir = tp ? -fl(c) : fl(c), tii = tp ? -fl(c + 1) : fl(c + 1);
+      const float tjr = tp ? -fl(c + 2) : fl(c + 2), tji = tp ? -fl(c + 3) : fl(c + 3);
+      const bool bi = (SM >> s) & 1u, bj = bi != ((KP & 1u) != 0u);
+      wir = bi ? -tir : tir;
+      wii = bi ? -tii : tii;
+      wjr = bj ? -tjr : tjr;
+      wji = bj ? -tji : tji;
+    } else {
+      const uint32_t sgi = tp ^ ((sm >> s) & 1u);
+      const uint32_t sgj = sgi ^ kpar;
+      wjr = sgj ? -fl(c + 2) : fl(c + 2);
+      wji = sgj ? -fl(c + 3) : fl(c + 3);
+      wir = sgi ? -fl(c) : fl(c);
+      wii = sgi ? -fl(c + 1) : fl(c + 1);
+    }
     // Im(conj(l) w a) = l.x (w.r a.y + w.i a.x) - l.y (w.r a.x - w.i a.y)
-    const float wjr = sgj ? -fl(c + 2) : fl(c + 2), wji = sgj ? -fl(c + 3) : fl(c + 3);
-    const float wir = sgi ? -fl(c) : fl(c), wii = sgi ? -fl(c + 1) : fl(c + 1);
     const float2 li = b[s], lj = b[s1], ai = a[s], aj = a[s1];
     const float qjr = wjr * aj.x - wji * aj.y, qji = wjr * aj.y + wji * aj.x;
     const float qir = wir * ai.x - wii * ai.y, qii = wir * ai.y + wii * ai.x;
@@ -787,9 +840,32 @@ __device__ __forceinline__ void red_group(const float2 (&a)[NSLOT], const float2
     vi = fmaf(lj.x, qii, vi);
     vi = fmaf(-lj.y, qir, vi);
   }
+  return vi;
+}
+
+template <int XR, typename IDX>
+__device__ __forceinline__ void red_group(const float2 (&a)[NSLOT], const float2 (&b)[NSLOT],
+                                          cw_t w, const PhaseGeo<IDX>& G, double* acc,
+                                          uint32_t nred) {
+  cw_t e = w + RED_HDR;
+  cw_t t = e + RED_ENT;
+  const float vi = group_part<XR, false, 0u, 0u, 0u, IDX>(a, b, G, ld_idx<IDX>(e + 1), t[0], t[1],
+                                                          e[3], t + 2);
   const double d = wave_sum_last_d((double)vi);
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
   if (lane == 63) acc[wave * nred + e[0]] += d;
+}
+
+// the group reduction of generated kernels
+template <int XR, uint32_t ACT, uint32_t SM, uint32_t KP, uint32_t SLOT, uint32_t NRED,
+          typename IDX>
+__device__ __forceinline__ void jred_group(const float2 (&a)[NSLOT], const float2 (&b)[NSLOT],
+                                           cw_t w, const PhaseGeo<IDX>& G, IDX z0, double* acc) {
+  const float vi =
+      group_part<XR, true, ACT, SM, KP, IDX>(a, b, G, z0, 0u, 0u, 0u, w + RED_HDR + RED_ENT + 2);
+  const double d = wave_sum_last_d((double)vi);
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  if (lane == 63) acc[wave * NRED + SLOT] += d;
 }
 
 template <int XR, int MODE, typename IDX>
@@ -801,7 +877,7 @@ __device__ __forceinline__ void red_dispatch_kind(const float2 (&a)[NSLOT],
   if constexpr (MODE == MODE_DUAL) {
     if (kind == RED_CROSS) red_cross<XR, IDX>(a, b, w, G, acc, nred);
     if constexpr (XR != 0)
-      if (kind == RED_GROUP) red_group<XR, -1, IDX>(a, b, w, G, acc, nred);
+      if (kind == RED_GROUP) red_group<XR, IDX>(a, b, w, G, acc, nred);
   } else {
     if constexpr (XR == 0) {
       red_x<0, RED_DIAG, MODE, IDX>(a, b, w, G, acc, nred);
@@ -944,25 +1020,30 @@ __device__ __forceinline__ void jit_geo(PhaseGeo<IDX>& G, IDX base, uint32_t tid
   G.gb = base | TM::pos(tpos);
 }
 
-template <int MODE, typename IDX>
+// USED: the slots the phase's steps read or write (phases made of rotation
+// groups touch only the pairs of their active classes); the others are
+// neither loaded nor stored.
+template <int MODE, uint32_t USED = 0xffffffffu, typename IDX>
 __device__ __forceinline__ void jit_load_slots(float2 (&a)[NSLOT], float2 (&b)[NSLOT],
                                                const float2* tA, const float2* tB,
                                                const PhaseGeo<IDX>& G) {
 #pragma unroll
   for (int s = 0; s < NSLOT; ++s) {
+    if (!((USED >> s) & 1u)) continue;
     const uint32_t l = G.bt ^ ls_of(G, s);
     a[s] = tA[l];
     if constexpr (MODE == MODE_DUAL) b[s] = tB[l];
   }
 }
 
-template <int MODE, typename IDX>
+template <int MODE, uint32_t USED = 0xffffffffu, typename IDX>
 __device__ __forceinline__ void jit_store_slots(const float2 (&a)[NSLOT], const float2 (&b)[NSLOT],
                                                 float2* tA, float2* tB, const PhaseGeo<IDX>& G) {
   uint32_t btw = G.bt;
   __asm__ volatile("" : "+v"(btw));
 #pragma unroll
   for (int s = 0; s < NSLOT; ++s) {
+    if (!((USED >> s) & 1u)) continue;
     const uint32_t l = btw ^ ls_of(G, s);
     tA[l] = a[s];
     if constexpr (MODE == MODE_DUAL) tB[l] = b[s];
@@ -1128,13 +1209,14 @@ __device__ __forceinline__ void jprot(float2 (&a)[NSLOT], float2 (&b)[NSLOT], cw
   if constexpr (MODE == MODE_DUAL) prot_apply1<XR, Q, IDX>(b, G, z, c, sn, xzpar);
 }
 
-template <int XR, int ACT, int MODE, typename IDX>
+// rotation group of generated kernels: x, z0 and the class/sign structure
+// fixed at compile time, the class constants read from the program
+template <int XR, uint32_t ACT, uint32_t SM, uint32_t KP, int MODE, typename IDX>
 __device__ __forceinline__ void jpgrp(float2 (&a)[NSLOT], float2 (&b)[NSLOT], cw_t w,
-                                      const PhaseGeo<IDX>& G) {
-  const IDX z0 = ld_idx<IDX>(w + 2);
-  const uint32_t kpar = w[4];
-  pgrp_apply1<XR, ACT, IDX>(a, G, z0, kpar, 0u, w + PG_HDR);
-  if constexpr (MODE == MODE_DUAL) pgrp_apply1<XR, ACT, IDX>(b, G, z0, kpar, 0u, w + PG_HDR);
+                                      const PhaseGeo<IDX>& G, IDX z0) {
+  pgrp_apply1<XR, true, ACT, SM, KP, IDX>(a, G, z0, 0u, 0u, 0u, w + PG_HDR);
+  if constexpr (MODE == MODE_DUAL)
+    pgrp_apply1<XR, true, ACT, SM, KP, IDX>(b, G, z0, 0u, 0u, 0u, w + PG_HDR);
 }
 
 template <int NWAVE, int NRED>

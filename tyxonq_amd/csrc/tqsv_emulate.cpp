@@ -232,7 +232,7 @@ void run_steps(const uint32_t* w, uint32_t nsteps, cf* a, cf* b, bool dual, uint
         break;
       }
       case OP_PGRP: {
-        const uint32_t xr = w[1], kpar = w[4], act = w[5];
+        const uint32_t xr = w[1], kpar = w[4], act = w[5], sm = w[6];
         const uint64_t z0 = w64(w + 2);
         const uint32_t lb = xr & (~xr + 1u), hi = xr & ~lb;
         for (int z = 0; z < narr; ++z) {
@@ -248,7 +248,7 @@ void run_steps(const uint32_t* w, uint32_t nsteps, cf* a, cf* b, bool dual, uint
             if (!((act >> cls) & 1u)) continue;
             const uint32_t* e = w + PG_HDR + PG_ENT * cls;
             const int s1 = s ^ (int)xr;
-            const uint32_t sgi = par64((gb ^ gs(s)) & z0), sgj = sgi ^ kpar;
+            const uint32_t sgi = par64(gb & z0) ^ ((sm >> s) & 1u), sgj = sgi ^ kpar;
             const float cm1 = fw(e);
             const cf ui = cf(fw(e + 1), fw(e + 2)) * (sgi ? -1.f : 1.f);
             const cf uj = cf(fw(e + 3), fw(e + 4)) * (sgj ? -1.f : 1.f);
@@ -280,7 +280,7 @@ void run_steps(const uint32_t* w, uint32_t nsteps, cf* a, cf* b, bool dual, uint
             if (!((act >> cls) & 1u)) continue;
             const uint32_t* c = tab + RGRP_ENT * cls;
             const int s1 = s ^ (int)xr;
-            const uint32_t sgi = par64((gb ^ gs(s)) & z0), sgj = sgi ^ kpar;
+            const uint32_t sgi = par64(gb & z0) ^ ((e[3] >> s) & 1u), sgj = sgi ^ kpar;
             const cf wi = cf(fw(c), fw(c + 1)) * (sgi ? -1.f : 1.f);
             const cf wj = cf(fw(c + 2), fw(c + 3)) * (sgj ? -1.f : 1.f);
             acc += (std::conj(b[s]) * wj * a[s1] + std::conj(b[s1]) * wi * a[s]).imag();

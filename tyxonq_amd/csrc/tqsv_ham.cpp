@@ -574,13 +574,18 @@ static void collect_grad(const tqsv_plan* p, const double* hred, int nparams, do
       grad[g.param] += scale * g.gcoef * g.red_scale * hred[g.red_slot];
       continue;
     }
-    const double re = hred[g.red_slot], im = hred[g.red_slot + 1];
+    // <lambda|P|psi> = i^k <lambda|X^x Z^z|psi>; the gradient takes its
+    // imaginary part: Im(c), -Re(c), -Im(c), Re(c) for k = 0..3.  Tile passes
+    // store only that part of c, single-op passes store re and im.
+    const double* h = hred + g.red_slot;
+    const double c_re = g.red_part == 1 ? 0.0 : h[0];
+    const double c_im = g.red_part == 0 ? 0.0 : g.red_part == 1 ? h[0] : h[1];
     double v;
     switch (g.k) {
-      case 0: v = im; break;
-      case 1: v = -re; break;
-      case 2: v = -im; break;
-      default: v = re; break;
+      case 0: v = c_im; break;
+      case 1: v = -c_re; break;
+      case 2: v = -c_im; break;
+      default: v = c_re; break;
     }
     grad[g.param] += scale * g.gcoef * g.red_scale * v;
   }

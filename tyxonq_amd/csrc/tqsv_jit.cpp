@@ -66,6 +66,43 @@ int pass_mode(uint32_t flags) {
   return (flags & PF_WRITE) ? MODE_GATE : MODE_RED;
 }
 
+// slots of the pairs (s, s ^ xr) whose class (the bits of s on xr above its
+// lowest bit) is in `act`
+uint32_t class_slots(uint32_t xr, uint32_t act) {
+  const uint32_t lb = xr & (~xr + 1u), hi = xr & ~lb;
+  uint32_t m = 0;
+  for (uint32_t s = 0; s < (uint32_t)NSLOT; ++s) {
+    if (s & lb) continue;
+    uint32_t cls = 0, k = 0;
+    for (int b = 0; b < RB; ++b)
+      if ((hi >> b) & 1u) {
+        if ((s >> b) & 1u) cls |= 1u << k;
+        ++k;
+      }
+    if ((act >> cls) & 1u) m |= (1u << s) | (1u << (s ^ xr));
+  }
+  return m;
+}
+
+// The slots a phase's steps read or write: rotation groups and group
+// reductions touch the pairs of their active classes, every other step all
+// slots.  ws = offset of the phase's first step in pw.
+uint32_t phase_slots(const uint32_t* pw, size_t ws, uint32_t nsteps) {
+  uint32_t m = 0;
+  for (uint32_t is = 0; is < nsteps; ++is) {
+    const uint32_t* w = pw + ws;
+    const uint32_t op = w[0] & 0xffffu;
+    if (op == OP_PGRP)
+      m |= class_slots(w[1], w[5]);
+    else if (op == OP_RED && w[2] == RED_GROUP)
+      m |= class_slots(w[1], w[RED_HDR + RED_ENT + 1]);
+    else
+      return 0xffffffffu;
+    ws += w[0] >> 16;
+  }
+  return m;
+}
+
 // Source of the kernel for one pass record (pw = the pass's program words).
 std::string gen_source(int tb, const uint32_t* pw, int n) {
   const uint32_t flags = pw[PH_FLAGS], L = pw[PH_L], m = pw[PH_M];
@@ -134,9 +171,11 @@ std::string gen_source(int tb, const uint32_t* pw, int n) {
     const uint32_t nsteps = ph[PS_NSTEPS];
     o << "    {\n      PhaseGeo<IDX> G;\n      jit_geo<TM, IDX";
     for (int j = 0; j < RB; ++j) o << ", " << ph[PS_R + j];
+    const uint32_t used = phase_slots(pw, at + PS_SIZE, nsteps);
+    const std::string um = used == 0xffffffffu ? "" : ", " + std::to_string(used) + "u";
     o << ">(G, base, tid);\n"
       << "      float2 a[NSLOT], b[NSLOT];\n"
-      << "      jit_load_slots<MODE>(a, b, tA, tB, G);\n";
+      << "      jit_load_slots<MODE" << um << ">(a, b, tA, tB, G);\n";
     size_t ws = at + PS_SIZE;
     for (uint32_t is = 0; is < nsteps; ++is) {
       const uint32_t* w = pw + ws;
@@ -173,9 +212,12 @@ std::string gen_source(int tb, const uint32_t* pw, int n) {
           o << "      jprot<" << xr << ", " << q << ", MODE>(a, b, " << P << ", G);\n";
           break;
         }
-        case OP_PGRP:
-          o << "      jpgrp<" << w[1] << ", " << w[5] << ", MODE>(a, b, " << P << ", G);\n";
+        case OP_PGRP: {
+          const uint64_t z0 = (uint64_t)w[2] | ((uint64_t)w[3] << 32);
+          o << "      jpgrp<" << w[1] << ", " << w[5] << "u, " << w[6] << "u, " << w[4] << "u, MODE>(a, b, "
+            << P << ", G, (IDX)" << z0 << "ull);\n";
           break;
+        }
         case OP_RED: {
           const uint32_t xr = w[1];
           if (racc) {
@@ -192,11 +234,19 @@ std::string gen_source(int tb, const uint32_t* pw, int n) {
             o << "      }\n";
             break;
           }
-          if (mode == MODE_DUAL && w[2] == RED_GROUP)
-            o << "      red_group<" << xr << ", " << w[RED_HDR + RED_ENT + 1] << ", IDX>(a, b, " << P
-              << ", G, acc, " << nred << ");\n";
-          else if (mode == MODE_DUAL)
-            o << "      red_cross<" << xr << ", IDX>(a, b, " << P << ", G, acc, " << nred << ");\n";
+          if (mode == MODE_DUAL && w[2] == RED_GROUP) {
+            const uint32_t* e = w + RED_HDR;
+            const uint64_t z0 = (uint64_t)e[1] | ((uint64_t)e[2] << 32);
+            o << "      jred_group<" << xr << ", " << e[RED_ENT + 1] << "u, " << e[3] << "u, "
+              << e[RED_ENT] << "u, " << e[0] << ", " << nred << ", IDX>(a, b, " << P << ", G, (IDX)"
+              << z0 << "ull, acc);\n";
+          }
+          else if (mode == MODE_DUAL) {
+            const uint32_t* e = w + RED_HDR;
+            const uint64_t zt = (uint64_t)e[1] | ((uint64_t)e[2] << 32);
+            o << "      jred_cross<" << xr << ", " << e[4] << ", " << e[3] << "u, " << e[0] << ", "
+              << nred << ", IDX>(a, b, G, (IDX)" << zt << "ull, acc);\n";
+          }
           else
             o << "      red_x<" << xr << ", " << (xr ? RED_PAIR : RED_DIAG) << ", MODE, IDX>(a, b, "
               << P << ", G, acc, " << nred << ");\n";
@@ -206,7 +256,7 @@ std::string gen_source(int tb, const uint32_t* pw, int n) {
       }
       ws += nw;
     }
-    if (write) o << "      jit_store_slots<MODE>(a, b, tA, tB, G);\n";
+    if (write) o << "      jit_store_slots<MODE" << um << ">(a, b, tA, tB, G);\n";
     o << "      __syncthreads();\n    }\n";
     at = ws;
   }

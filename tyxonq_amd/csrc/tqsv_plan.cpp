@@ -68,7 +68,7 @@ int gate_cost(const Gate& g) {
     case GK_MAT2: return g.cmask ? 20 : 16;
     case GK_DIAG2: return 1;
     case GK_PROT: return 8;
-    case GK_RED: return g.nmem > 0 ? 6 : 10;
+    case GK_RED: return g.nmem > 0 ? 6 : 4;
     case GK_PGRP: {
       PgrpClass cls[kMaxGroup];
       const int nc = pgrp_classes(g, cls);
@@ -798,6 +798,7 @@ void emit_phase(std::vector<Gate>& gates, const std::vector<int>& sel,
         e.u64(g.z);
         e.u((uint32_t)(popc(g.x & g.z) & 1));
         e.u(act);
+        e.u(smask_of(slotmask(g.z)));
         for (int c = 0; c < nc; ++c) {
           e.f((float)cls[c].cm1);
           e.c(cls[c].ui);
@@ -838,7 +839,7 @@ void emit_phase(std::vector<Gate>& gates, const std::vector<int>& sel,
           e.u(1);
           e.u(nred);
           e.u64(g.z);
-          e.u(0);
+          e.u(smask_of(slotmask(g.z)));
           e.u(1);
           e.u((uint32_t)(popc(g.x & g.z) & 1));
           e.u(act);
@@ -851,27 +852,29 @@ void emit_phase(std::vector<Gate>& gates, const std::vector<int>& sel,
           }
           e.end(at);
           g.red_slot = (int)(red_base + nred);
+          g.red_part = 1;
           g.red_scale = std::exp(2.0 * P.c0.real());
           nred += 1;
           break;
         }
+        // the gradient needs one part of <lambda|P|psi> = i^k <lambda|X^x Z^z|psi>:
+        // the imaginary part for even k, the real part for odd k
         const uint32_t xr = slotmask(g.x);
         const uint32_t zs = slotmask(g.z);
-        const uint32_t sm = smask_of(zs);
+        const uint32_t part = (g.k & 1) ? 0u : 1u;
         const size_t at = e.begin(OP_RED);
         e.u(xr);
         e.u(RED_CROSS);
-        e.u(2);
-        for (uint32_t part = 0; part < 2; ++part) {
-          e.u(nred + part);
-          e.u64(g.z);
-          e.u(sm);
-          e.u(part);
-        }
+        e.u(1);
+        e.u(nred);
+        e.u64(g.z);
+        e.u(smask_of(zs));
+        e.u(part);
         e.end(at);
         g.red_slot = (int)(red_base + nred);
+        g.red_part = (int)part;
         g.red_scale = std::exp(2.0 * P.c0.real());
-        nred += 2;
+        nred += 1;
         break;
       }
     }
@@ -1268,13 +1271,13 @@ int select_passes(std::vector<Gate>& gates, const SchedOpts& so, std::vector<Pas
       const Gate& g = gates[gi];
       if (conflicts(g, blkN, blkD) || !fusable(g) || (g.nb & ~tilebits) ||
           (!chosen.empty() && g.cost > budget) ||
-          (g.kind == GK_RED && nred_pass + 2 > red_cap)) {
+          (g.kind == GK_RED && nred_pass + 1 > red_cap)) {
         block(g, blkN, blkD);
         continue;
       }
       chosen.push_back(gi);
       budget -= g.cost;
-      if (g.kind == GK_RED) nred_pass += g.nmem > 0 ? 1 : 2;
+      if (g.kind == GK_RED) nred_pass += 1;
     }
     if (chosen.empty()) return fail(TQSV_EINVAL, "planner made no progress (internal error)");
     // 3) phases
@@ -1351,6 +1354,7 @@ int emit_passes(std::vector<Gate>& gates, const SchedOpts& so, std::vector<PassP
         pp.nred = 2;
         pp.red_base = red_total;
         g.red_slot = (int)red_total;
+        g.red_part = -1;
         g.red_scale = 1.0;
         red_total += 2;
       }

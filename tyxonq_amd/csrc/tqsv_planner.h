@@ -43,6 +43,9 @@ struct Gate {
   uint64_t gx = 0, gz = 0;
   double gcoef = 0.0;
   int red_slot = -1;
+  // which part of <lambda|P|psi> the slot holds: -1 two slots (re, im), 0 one
+  // slot with the real part, 1 one slot with the imaginary part
+  int red_part = -1;
   double red_scale = 1.0;  // true <lambda|P|psi> = red_scale * (value read from the sweep)
   // rotation form (tan 0 / cot 1) chosen when the gate was last emitted; for
   // plans on generated kernels it is kept across parameter updates while
@@ -132,8 +135,9 @@ struct SchedOpts {
 void classify_gate(Gate& g);
 
 // Plan passes over `gates` (in order).  Appends the programs to `prog`.
-// Reduction gates get their output slot (gates[i].red_slot, two doubles:
-// re, im) allocated from `red_total`.
+// Reduction gates get their output slots (gates[i].red_slot: one double, the
+// part of <lambda|P|psi> the gradient needs, in tile passes; two doubles, re
+// and im, in single-op passes) allocated from `red_total`.
 int schedule(std::vector<Gate>& gates, const SchedOpts& so, std::vector<PassPlan>& passes,
              std::vector<uint32_t>& prog, uint32_t& red_total);
 

@@ -93,14 +93,16 @@ enum : uint32_t {
   // {slot, zt lo, zt hi, smask, part} = 5 words.
   //  kind 0: pairs s (lowest set bit of xr clear): v = conj(a[s^xr]) a[s]
   //  kind 1: all s: v = |a[s]|^2 (xr = 0)
-  //  kind 2: all s: v = conj(b[s]) a[s^xr]  (b = lambda, dual passes)
+  //  kind 2: all s: v = conj(b[s]) a[s^xr]  (b = lambda, dual passes; one
+  //    entry: the one part of <lambda|P|psi> the gradient needs)
   // term value += sum_s (-1)^{smask bit s} (-1)^{|gb & zt|} part(v)
   //  kind 3 (dual passes, one entry; zt = z0 of a rotation group): the
   //    imaginary part of <lambda| G |psi> for the group generator G =
   //    sum_r g_r P_r, summed over the pairs (i, j = i ^ x) of the active
   //    classes: conj(b_i) sg(j) V_j a_j + conj(b_j) sg(i) V_i a_i, with
-  //    sg(k) = (-1)^{|k & z0|}.  After the entry: [kpar] [active class mask]
-  //    then per class {V_i re, V_i im, V_j re, V_j im}.
+  //    sg(k) = (-1)^{|k & z0|} (the entry's smask: its register-bit part).
+  //    After the entry: [kpar] [active class mask] then per class
+  //    {V_i re, V_i im, V_j re, V_j im}.
   OP_RED = 10,
   // real scale: a[s] *= f for all s.  [1]=f (float)
   OP_SCL = 11,
@@ -118,9 +120,11 @@ enum : uint32_t {
   //   new_i = a_i + cm1 a_i + sg(j) u_j a_j,  new_j = a_j + cm1 a_j + sg(i) u_i a_i
   // with sg(k) = (-1)^{|k & z0|} and the class constants of the pair's
   // pattern on x (cm1 = cos - 1, u = -i sin/w times the generator entry).
-  // [1]=xr [2..4)=z0 (lo,hi) [4]=|x & z0| mod 2 [5]=active class mask, then
-  // per class (pattern of the x bits above the lowest one, compressed)
-  // {cm1, u_i re, u_i im, u_j re, u_j im}; inactive classes are the identity.
+  // [1]=xr [2..4)=z0 (lo,hi) [4]=|x & z0| mod 2 [5]=active class mask
+  // [6]=slot sign mask (bit s = parity of slot s's register bits & z0; the
+  // thread's bits give the rest of sg), then per class (pattern of the x bits
+  // above the lowest one, compressed) {cm1, u_i re, u_i im, u_j re, u_j im};
+  // inactive classes are the identity.
   OP_PGRP = 13,
 };
 
@@ -146,7 +150,7 @@ constexpr int THR_ENT = 4;
 constexpr int TAB_OFF = 4;
 constexpr int RED_HDR = 4;
 constexpr int RED_ENT = 5;
-constexpr int PG_HDR = 6;
+constexpr int PG_HDR = 7;
 constexpr int PG_ENT = 5;
 
 }  // namespace tqsv
