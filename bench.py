@@ -336,22 +336,7 @@ def run_adjoint(args, steps=None, warmup=None, n=None, layers=None) -> dict:
         e, g = plan.adjoint(ham, psi, lam)
     _sync()
     adj_s = (time.perf_counter() - t0) / steps
-    # the dominant kernel: the dual pass of the reverse sweep (psi and lambda
-    # read and written once per launch), timed alone through the VJP entry,
-    # which runs exactly the reverse sweep
-    rinfo = plan.info()
-    nrev = max(int(rinfo["reverse_passes"]), 1)
-    ev0, ev1 = _events()
-    vjp_ms = []
-    for _ in range(5):
-        ev0.record()
-        plan.vjp(psi, lam, npar)
-        ev1.record()
-        ev1.synchronize()
-        vjp_ms.append(ev0.elapsed_time(ev1))
-    dual_ms = float(np.median(vjp_ms)) / nrev
-    dual_bytes = 32 * (1 << n)
-    dual_achieved = dual_bytes / (dual_ms * 1e-3) / 1e9
+    roof = _dual_pass_roofline(plan, psi, lam, n)
     gold = _golden("adjoint22")
     vs_c128 = None
     if gold and (gold["qubits"], gold["layers"], gold["seed"]) == (n, layers, args.seed):
@@ -436,14 +421,37 @@ def run_adjoint(args, steps=None, warmup=None, n=None, layers=None) -> dict:
         "max_abs_grad_diff_vs_shift_sample": float(np.max(np.abs(g[:k] - grad_ps))) if k else None,
         "vs_complex128": vs_c128,
         "build_s": round(build_s, 1),
-        "roofline": {"bound": "hbm", "achieved": round(dual_achieved, 1), "peak": HBM_PEAK_GBPS,
-                     "unit": "GB/s", "frac": round(dual_achieved / HBM_PEAK_GBPS, 4),
-                     "traffic": None,
-                     "kernel": "tqsv_adjoint_pass (dual pass: psi and lambda read + written)",
-                     "bytes_per_launch": dual_bytes, "avg_launch_ms": round(dual_ms, 5),
-                     "launches_per_sweep": nrev},
+        "roofline": roof,
         "cpu_baseline": None,
     }
+
+
+def _dual_pass_roofline(plan, psi, lam, n: int) -> dict:
+    """The dominant kernel of an energy+gradient: the dual pass of the reverse
+    sweep (psi and lambda read and written once per launch, 32 bytes per
+    amplitude), timed alone through the VJP entry, which runs exactly the
+    reverse sweep; the average over its passes, per-pass partial sums
+    included."""
+    info = plan.info()
+    nrev = max(int(info["reverse_passes"]), 1)
+    ev0, ev1 = _events()
+    vjp_ms = []
+    for _ in range(5):
+        ev0.record()
+        plan.vjp(psi, lam, plan.nparams)
+        ev1.record()
+        ev1.synchronize()
+        vjp_ms.append(ev0.elapsed_time(ev1))
+    dual_ms = float(np.median(vjp_ms)) / nrev
+    dual_bytes = 32 * (1 << n)
+    achieved = dual_bytes / (dual_ms * 1e-3) / 1e9
+    gen = bool(info.get("reverse_generated", False))
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+            "kernel": ("tqsv_adjoint_pass" if gen else "k_pass<8, 2, ...>")
+            + " (dual pass: psi and lambda read + written)",
+            "bytes_per_launch": dual_bytes, "avg_launch_ms": round(dual_ms, 5),
+            "launches_per_sweep": nrev}
 
 
 # ---- config 1: 12-qubit hardware-efficient ansatz, TFIM --------------------------
@@ -524,13 +532,23 @@ def run_ucc(args) -> dict:
     thetas = [rng.normal(0.0, 0.05, size=npar) for _ in range(args.warmup + args.steps)]
     x, z, c = PauliSum(n, terms).to_masks()
     ham = nat.Ham(n, x, z, c)
+    # generated kernels unless --jit -1: about 160 pass kernels, compiled once
+    # per plan structure (seconds on 16 host threads), then reused for every
+    # parameter update
+    jit = args.jit if args.jit else 1
     plan = nat.Plan(n, ans.ops(thetas[0]), tile_bits=args.tile_bits or 13, max_cost=100000,
-                    jit=args.jit)
+                    jit=jit)
     psi = torch.empty(1 << n, dtype=torch.complex64, device=nat.device())
     lam = torch.empty_like(psi)
     nat.init_basis(psi, 0)
-    plan.adjoint(ham, psi, lam)
+    e0, g0 = plan.adjoint(ham, psi, lam)
     build_s = time.perf_counter() - t0
+    gold = _golden("ucc20")
+    vs_c128 = None
+    if gold and (gold["spin_orbitals"], gold["seed"], gold["theta_sigma"]) == (n, args.seed, 0.05):
+        vs_c128 = {"abs_dE": abs(e0 - gold["energy"]),
+                   "max_abs_dgrad": float(np.max(np.abs(g0 - np.asarray(gold["grad"])))),
+                   "params_compared": len(gold["grad"])}
 
     def step(th):
         plan.update(ans.ops(th))
@@ -547,6 +565,7 @@ def run_ucc(args) -> dict:
         lat.append(time.perf_counter() - t0)
     lat = np.array(lat)
     info = plan.info()
+    roof = _dual_pass_roofline(plan, psi, lam, n)
 
     # the same energy+gradient through Circuit + torch.autograd (the drop-in surface)
     eng = StatevectorEngine(backend_name="pytorch")
@@ -581,11 +600,13 @@ def run_ucc(args) -> dict:
         "grad_norm": float(np.linalg.norm(gr)),
         "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
         "gate_passes": info["passes"],
+        "generated_kernels": bool(info.get("generated", False)),
+        "vs_complex128": vs_c128,
         "autograd_path": {"ms": round(auto_s * 1e3, 1),
                           "abs_dE_vs_direct": abs(float(ea.detach()) - e) if args.steps else None,
                           "max_abs_dgrad_vs_direct": float(np.max(np.abs(tt.grad.numpy() - gr)))},
         "build_s": round(build_s, 1),
-        "roofline": None,
+        "roofline": roof,
         "cpu_baseline": None,
     }
 

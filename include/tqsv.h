@@ -182,6 +182,8 @@ typedef struct tqsv_plan_info {
   int64_t program_bytes;   /* device program size */
   int32_t reverse_passes;  /* dual passes of the adjoint / VJP sweep (0 until first built) */
   int32_t reverse_single;  /* reverse items run as single-op launches */
+  int32_t generated;       /* forward passes run on generated (run-time compiled) kernels */
+  int32_t reverse_generated; /* the same for the reverse sweep */
 } tqsv_plan_info;
 
 typedef struct tqsv_plan tqsv_plan;
@@ -203,6 +205,16 @@ int tqsv_plan_get_info(const tqsv_plan* plan, tqsv_plan_info* info);
  * tqsv_plan_create would build.  For planning inspection and tests. */
 int tqsv_plan_inspect(int n, const tqsv_op* ops, int nops, const tqsv_plan_opts* opts,
                       tqsv_plan_info* info);
+
+/* Plan on the host and compile the generated kernel of every tile pass
+ * (with adjoint != 0 also those of the reverse sweep) without touching a
+ * device: the compiler keeps the code objects in its disk cache, so that a
+ * later tqsv_plan_create with generated kernels (opts->jit = 1) of the same
+ * circuit structure loads them instead of compiling.  Needs no GPU.
+ * *kernels (may be NULL) = distinct kernels compiled.  The reference has no
+ * counterpart (it compiles nothing); for serving start-up and tests. */
+int tqsv_plan_precompile(int n, const tqsv_op* ops, int nops, const tqsv_plan_opts* opts,
+                         int adjoint, int* kernels);
 
 /* Plan on the host and execute the pass programs on a HOST complex64 state
  * (2^n interleaved float pairs, in place, n <= 26) with the device kernel's

@@ -140,7 +140,10 @@ def test_adjoint22_energy_and_full_gradient(gpu):
     np.testing.assert_allclose(grad, gref, atol=1e-5)
 
 
-def test_ucc20_energy_and_full_gradient(gpu):
+@pytest.mark.parametrize("jit", [0, 1])
+def test_ucc20_energy_and_full_gradient(gpu, jit):
+    """jit 0: the automatic choice (interpreting kernels at 20 qubits); jit 1:
+    generated kernels, with the rotation groups' sparse phases."""
     import torch
 
     from tyxonq_amd import _native as nat
@@ -158,13 +161,15 @@ def test_ucc20_energy_and_full_gradient(gpu):
     th = np.random.default_rng(g["seed"]).normal(0.0, g["theta_sigma"], size=ans.nparams)
     x, z, c = PauliSum(n, terms).to_masks()
     ham = nat.Ham(n, x, z, c)
-    plan = nat.Plan(n, ans.ops(th), max_cost=100000)
+    plan = nat.Plan(n, ans.ops(th), max_cost=100000, jit=jit)
     psi = torch.empty(1 << n, dtype=torch.complex64, device=nat.device())
     lam = torch.empty_like(psi)
     nat.init_basis(psi, 0)
     e, grad = plan.adjoint(ham, psi, lam)
+    info = plan.info()
+    assert (info["generated"], info["reverse_generated"]) == (jit, jit)
     gref = np.asarray(g["grad"])
-    print(f"ucc20 |dE| = {abs(e - g['energy']):.3e}, "
+    print(f"ucc20 jit={jit} |dE| = {abs(e - g['energy']):.3e}, "
           f"max|dgrad| = {np.max(np.abs(grad - gref)):.3e} (max|g| {np.max(np.abs(gref)):.3f})")
     np.testing.assert_allclose(gref[g["fd_indices"]], g["fd_grad"], atol=1e-7)
     assert abs(e - g["energy"]) <= 1e-6

@@ -67,6 +67,8 @@ class PlanInfo(ctypes.Structure):
         ("program_bytes", ctypes.c_int64),
         ("reverse_passes", ctypes.c_int32),
         ("reverse_single", ctypes.c_int32),
+        ("generated", ctypes.c_int32),
+        ("reverse_generated", ctypes.c_int32),
     ]
 
 
@@ -114,6 +116,8 @@ _SIGS = {
     "tqsv_plan_get_info": (_I, [_P, ctypes.POINTER(PlanInfo)]),
     "tqsv_plan_inspect": (_I, [_I, _P, _I, ctypes.POINTER(PlanOpts), ctypes.POINTER(PlanInfo)]),
     "tqsv_plan_emulate": (_I, [_I, _P, _I, ctypes.POINTER(PlanOpts), _P]),
+    "tqsv_plan_precompile": (_I, [_I, _P, _I, ctypes.POINTER(PlanOpts), _I,
+                                  ctypes.POINTER(ctypes.c_int)]),
     "tqsv_plan_run": (_I, [_P, _P, _P]),
     "tqsv_plan_destroy": (None, [_P]),
     "tqsv_apply_fused": (_I, [_P, _I, _P, _I, ctypes.POINTER(PlanOpts), _P]),
@@ -366,6 +370,21 @@ def inspect(n: int, ops: np.ndarray, *, tile_bits: int = 0, min_low_bits: int = 
     _check(load().tqsv_plan_inspect(int(n), ops.ctypes.data if len(ops) else None, len(ops),
                                     ctypes.byref(o), ctypes.byref(inf)))
     return {f: getattr(inf, f) for f, _ in PlanInfo._fields_}
+
+
+def precompile(n: int, ops: np.ndarray, *, adjoint: bool = False, tile_bits: int = 0,
+               min_low_bits: int = 0, max_cost: int = 0, fuse: bool = True,
+               min_qubits: int = 0) -> int:
+    """Plan on the host and compile the generated kernels of the plan (and,
+    with adjoint=True, of its reverse sweep) into the compiler's disk cache,
+    without a GPU, so that a later Plan(..., jit=1) of the same structure
+    starts without compiling.  Returns the number of distinct kernels."""
+    ops = np.ascontiguousarray(ops, dtype=OP_DTYPE)
+    o = _opts(tile_bits, min_low_bits, max_cost, fuse, 1, min_qubits)
+    k = ctypes.c_int(0)
+    _check(load().tqsv_plan_precompile(int(n), ops.ctypes.data if len(ops) else None, len(ops),
+                                       ctypes.byref(o), 1 if adjoint else 0, ctypes.byref(k)))
+    return int(k.value)
 
 
 def emulate(n: int, ops: np.ndarray, state: np.ndarray, *, tile_bits: int = 0,

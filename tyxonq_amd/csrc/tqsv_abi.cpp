@@ -334,6 +334,8 @@ int tqsv_plan_get_info(const tqsv_plan* p, tqsv_plan_info* info) {
   for (const PassPlan& pp : p->rpasses) (pp.tile ? rt : rs) += 1;
   info->reverse_passes = rt;
   info->reverse_single = rs;
+  info->generated = p->fgen ? 1 : 0;
+  info->reverse_generated = p->rgen ? 1 : 0;
   return 0;
 }
 
@@ -368,6 +370,56 @@ int tqsv_plan_inspect(int n, const tqsv_op* ops, int nops, const tqsv_plan_opts*
   }
 }
 
+int tqsv_plan_precompile(int n, const tqsv_op* ops, int nops, const tqsv_plan_opts* opts,
+                         int adjoint, int* kernels) {
+  try {
+    if (n < 1 || n > 34) return fail(TQSV_EINVAL, "tqsv_plan_precompile: n must be in [1, 34]");
+    if (nops < 0 || (nops > 0 && !ops)) return fail(TQSV_EINVAL, "tqsv_plan_precompile: bad op array");
+    if (opts && (opts->min_qubits < 0 || opts->min_qubits > 13))
+      return fail(TQSV_EINVAL, "tqsv_plan_precompile: min_qubits must be in [0, 13]");
+    SchedOpts so;
+    if (opts) {
+      so.tile_bits = opts->tile_bits > 0 ? opts->tile_bits : 13;
+      so.min_low = opts->min_low_bits > 0 ? opts->min_low_bits : 5;
+      so.max_cost = opts->max_cost > 0 ? opts->max_cost : 0;
+      so.fuse = opts->fuse != 0;
+    }
+    const int pn = plan_qubits(n, so.fuse, opts ? opts->min_qubits : 0);
+    so.n = pn;
+    std::vector<Gate> g;
+    int rc = gates_from_ops(n, ops, nops, g);
+    if (rc) return rc;
+    merge_and_classify(g);
+    std::vector<PassPlan> passes;
+    std::vector<uint32_t> prog;
+    uint32_t red_total = 0;
+    rc = schedule(g, so, passes, prog, red_total);
+    if (rc) return rc;
+    int total = 0, k = 0;
+    rc = jit_compile_only(passes, prog, pn, &k);
+    if (rc) return rc;
+    total += k;
+    if (adjoint) {
+      std::vector<Gate> r;
+      reverse_gates(g, r);
+      SchedOpts ro = so;
+      ro.dual = true;
+      std::vector<PassPlan> rpasses;
+      std::vector<uint32_t> rprog;
+      red_total = 0;
+      rc = schedule(r, ro, rpasses, rprog, red_total);
+      if (rc) return rc;
+      rc = jit_compile_only(rpasses, rprog, pn, &k);
+      if (rc) return rc;
+      total += k;
+    }
+    if (kernels) *kernels = total;
+    return 0;
+  } catch (const std::bad_alloc&) {
+    return fail(TQSV_ENOMEM, "tqsv_plan_precompile: out of host memory");
+  }
+}
+
 int tqsv_plan_run(tqsv_plan* p, void* psi, void* stream) {
   if (!p || !psi) return fail(TQSV_EINVAL, "tqsv_plan_run: NULL argument");
   hipStream_t st = S(stream);
@@ -387,6 +439,7 @@ void tqsv_plan_destroy(tqsv_plan* p) {
   (void)hipFree(p->drprog);
   (void)hipFree(p->dpartials);
   (void)hipFree(p->dred);
+  (void)hipFree(p->denergy);
   (void)hipFree(p->pad_a);
   (void)hipFree(p->pad_b);
   delete p;

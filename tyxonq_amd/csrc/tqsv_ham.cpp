@@ -433,7 +433,7 @@ static size_t count_tile(const std::vector<PassPlan>& passes) {
 }
 
 int plan_build_forward(tqsv_plan* p, const tqsv_op* ops, int nops, hipStream_t st) {
-  std::vector<Gate> g;
+  std::vector<Gate>& g = p->gwork;
   int rc = gates_from_ops(p->nu, ops, nops, g);
   if (rc) return rc;
   merge_and_classify(g);
@@ -447,8 +447,10 @@ int plan_build_forward(tqsv_plan* p, const tqsv_op* ops, int nops, hipStream_t s
     rc = select_passes(g, p->so, sel);
     if (rc) return rc;
   }
-  std::vector<PassPlan> passes;
-  std::vector<uint32_t> prog;
+  std::vector<PassPlan>& passes = p->pwork;
+  std::vector<uint32_t>& prog = p->progwork;
+  passes.clear();
+  prog.clear();
   uint32_t red_total = 0;
   rc = emit_passes(g, p->so, sel, passes, prog, red_total);
   if (rc) return rc;
@@ -477,11 +479,10 @@ static int ensure(double** d, size_t* cap, size_t need) {
   return 0;
 }
 
-int plan_build_adjoint(tqsv_plan* p, hipStream_t st) {
-  if (p->adj_built) return 0;
-  std::vector<Gate> r;
-  for (int i = (int)p->gates.size() - 1; i >= 0; --i) {
-    const Gate& g = p->gates[i];
+void reverse_gates(const std::vector<Gate>& gates, std::vector<Gate>& r) {
+  r.clear();
+  for (int i = (int)gates.size() - 1; i >= 0; --i) {
+    const Gate& g = gates[i];
     bool shared = g.kind == GK_PGRP && g.nmem > 0 && g.mp[0] >= 0;
     for (int mi = 1; shared && mi < g.nmem; ++mi) shared = g.mp[mi] == g.mp[0];
     if (shared) {
@@ -522,6 +523,12 @@ int plan_build_adjoint(tqsv_plan* p, hipStream_t st) {
     }
     r.push_back(dagger(g));
   }
+}
+
+int plan_build_adjoint(tqsv_plan* p, hipStream_t st) {
+  if (p->adj_built) return 0;
+  std::vector<Gate>& r = p->rwork;
+  reverse_gates(p->gates, r);
   const bool same = same_structure(r, p->rgates) && !p->rsel.empty();
   if (p->rgen && r.size() == p->rgates.size())
     for (size_t i = 0; i < r.size(); ++i) r[i].rot_form = p->rgates[i].rot_form;
@@ -535,8 +542,10 @@ int plan_build_adjoint(tqsv_plan* p, hipStream_t st) {
     rc = select_passes(r, so, sel);
     if (rc) return rc;
   }
-  std::vector<PassPlan> passes;
-  std::vector<uint32_t> prog;
+  std::vector<PassPlan>& passes = p->rpwork;
+  std::vector<uint32_t>& prog = p->rprogwork;
+  passes.clear();
+  prog.clear();
   uint32_t red_total = 0;
   rc = emit_passes(r, so, sel, passes, prog, red_total);
   if (rc) return rc;
@@ -763,7 +772,8 @@ int tqsv_adjoint(tqsv_plan* p, tqsv_ham* h, void* psi, void* lam, int nparams, d
       return fail(TQSV_EINVAL, "tqsv_adjoint: nparams smaller than the largest parameter index + 1");
     if (psi == lam) return fail(TQSV_EINVAL, "tqsv_adjoint: lam must not alias psi");
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    int rc = plan_build_adjoint(p, st);
+    const uint32_t eg = reduce_grid(p->n);
+    int rc = ensure(&p->denergy, &p->denergy_cap, 2 * (size_t)eg + 2);
     if (rc) return rc;
     const bool pad = p->n > p->nu;
     float2* a = pad ? p->pad_a : reinterpret_cast<float2*>(psi);
@@ -778,8 +788,15 @@ int tqsv_adjoint(tqsv_plan* p, tqsv_ham* h, void* psi, void* lam, int nparams, d
     if (rc) return rc;
     TQSV_HIP(k_ham_apply(a, b, h->n, h->ngroups, h->d_gx, h->d_gstart, h->d_tz, h->d_tc, st));
     // E = Re <psi|H psi>
-    TQSV_HIP(k_pauli_dot(a, b, p->n, 0, 0, p->dpartials, st));
-    TQSV_HIP(k_final_sum(p->dpartials, reduce_grid(p->n), 2, p->dred + p->nred_adj, st));
+    TQSV_HIP(k_pauli_dot(a, b, p->n, 0, 0, p->denergy, st));
+    TQSV_HIP(k_final_sum(p->denergy, eg, 2, p->denergy + 2 * (size_t)eg, st));
+    // the reverse programs are built on the host while the device runs the
+    // forward sweep and H|psi> (after a parameter update they are re-emitted)
+    rc = plan_build_adjoint(p, st);
+    if (rc) return rc;
+    // the energy joins the reverse sweep's slots (after them), read back with them
+    TQSV_HIP(hipMemcpyAsync(p->dred + p->nred_adj, p->denergy + 2 * (size_t)eg, 2 * sizeof(double),
+                            hipMemcpyDeviceToDevice, st));
     std::vector<double> hred((size_t)p->nred_adj + 2);
     rc = reverse_and_collect(p, a, b, nparams, grad, 1.0, hred.data(), st);
     if (rc) return rc;

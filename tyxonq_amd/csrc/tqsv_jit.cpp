@@ -372,6 +372,36 @@ int jit_policy(int requested, int n, size_t tile_passes, size_t max_auto_passes)
 
 std::string jit_source(int tb, const uint32_t* pass_words, int n) { return gen_source(tb, pass_words, n); }
 
+int jit_compile_only(const std::vector<PassPlan>& passes, const std::vector<uint32_t>& prog, int n,
+                     int* compiled) {
+  std::vector<std::string> srcs;
+  {
+    std::unordered_map<std::string, int> seen;
+    for (const PassPlan& pp : passes) {
+      if (!pp.tile) continue;
+      std::string src = gen_source(pp.tb, prog.data() + pp.prog_off, n);
+      if (seen.emplace(src, 1).second) srcs.push_back(std::move(src));
+    }
+  }
+  std::vector<std::string> logs(srcs.size());
+  std::vector<char> ok(srcs.size(), 0);
+  std::atomic<size_t> next{0};
+  auto work = [&]() {
+    std::vector<char> code;
+    for (size_t j; (j = next.fetch_add(1)) < srcs.size();) ok[j] = compile_source(srcs[j], code, logs[j]) ? 1 : 0;
+  };
+  const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  const size_t nthr = std::min<size_t>(hw, srcs.size());
+  std::vector<std::thread> th;
+  for (size_t t = 1; t < nthr; ++t) th.emplace_back(work);
+  work();
+  for (auto& t : th) t.join();
+  for (size_t j = 0; j < srcs.size(); ++j)
+    if (!ok[j]) return fail(TQSV_EDEVICE, "run-time compilation of a pass kernel failed: " + logs[j]);
+  if (compiled) *compiled = (int)srcs.size();
+  return 0;
+}
+
 int jit_prepare(std::vector<PassPlan>& passes, const std::vector<uint32_t>& prog, int n) {
   int dev = 0;
   TQSV_HIP(hipGetDevice(&dev));

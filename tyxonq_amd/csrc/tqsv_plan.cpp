@@ -71,7 +71,7 @@ int gate_cost(const Gate& g) {
     case GK_RED: return g.nmem > 0 ? 6 : 4;
     case GK_PGRP: {
       PgrpClass cls[kMaxGroup];
-      const int nc = pgrp_classes(g, cls);
+      const int nc = pgrp_classes(g, cls, false);
       int act = 0;
       for (int c = 0; c < nc; ++c) act += cls[c].active ? 1 : 0;
       return std::max(1, (9 * act + nc - 1) / nc);
@@ -916,19 +916,22 @@ int gates_from_ops(int n, const tqsv_op* ops, int nops, std::vector<Gate>& out) 
   const uint64_t full = lowmask(n);
   for (int i = 0; i < nops; ++i) {
     const tqsv_op& o = ops[i];
-    const std::string where = "op " + std::to_string(i) + ": ";
-    Gate g;
+    // the message prefix is built only for a failing op
+    auto bad = [i](int code, const std::string& msg) {
+      return fail(code, "op " + std::to_string(i) + ": " + msg);
+    };
+    Gate& g = out.emplace_back();
     g.src = i;
     g.param = o.param;
     g.gx = o.gx;
     g.gz = o.gz;
     g.gcoef = o.gcoef;
     if ((o.cmask & ~full) || (o.cval & ~o.cmask))
-      return fail(TQSV_EINVAL, where + "control mask out of range");
+      return bad(TQSV_EINVAL, "control mask out of range");
     switch (o.kind) {
       case TQSV_OP_MAT1:
-        if (o.t[0] < 0 || o.t[0] >= n) return fail(TQSV_EINVAL, where + "target bit out of range");
-        if (o.cmask & bitm(o.t[0])) return fail(TQSV_EINVAL, where + "target is also a control");
+        if (o.t[0] < 0 || o.t[0] >= n) return bad(TQSV_EINVAL, "target bit out of range");
+        if (o.cmask & bitm(o.t[0])) return bad(TQSV_EINVAL, "target is also a control");
         g.kind = GK_MAT1;
         g.t0 = o.t[0];
         g.cmask = o.cmask;
@@ -937,10 +940,10 @@ int gates_from_ops(int n, const tqsv_op* ops, int nops, std::vector<Gate>& out) 
         break;
       case TQSV_OP_MAT2:
         if (o.t[0] < 0 || o.t[0] >= n || o.t[1] < 0 || o.t[1] >= n)
-          return fail(TQSV_EINVAL, where + "target bit out of range");
-        if (o.t[0] == o.t[1]) return fail(TQSV_EINVAL, where + "targets must differ");
+          return bad(TQSV_EINVAL, "target bit out of range");
+        if (o.t[0] == o.t[1]) return bad(TQSV_EINVAL, "targets must differ");
         if (o.cmask & (bitm(o.t[0]) | bitm(o.t[1])))
-          return fail(TQSV_EINVAL, where + "target is also a control");
+          return bad(TQSV_EINVAL, "target is also a control");
         g.kind = GK_MAT2;
         g.t0 = o.t[0];
         g.t1 = o.t[1];
@@ -949,8 +952,8 @@ int gates_from_ops(int n, const tqsv_op* ops, int nops, std::vector<Gate>& out) 
         for (int j = 0; j < 16; ++j) g.m[j] = cd(o.m[2 * j], o.m[2 * j + 1]);
         break;
       case TQSV_OP_PROT:
-        if ((o.xmask | o.zmask) & ~full) return fail(TQSV_EINVAL, where + "Pauli mask out of range");
-        if (o.cmask) return fail(TQSV_EINVAL, where + "Pauli rotations take no controls");
+        if ((o.xmask | o.zmask) & ~full) return bad(TQSV_EINVAL, "Pauli mask out of range");
+        if (o.cmask) return bad(TQSV_EINVAL, "Pauli rotations take no controls");
         g.kind = GK_PROT;
         g.x = o.xmask;
         g.z = o.zmask;
@@ -962,19 +965,18 @@ int gates_from_ops(int n, const tqsv_op* ops, int nops, std::vector<Gate>& out) 
           g.gz = g.z;
         }
         break;
-      default: return fail(TQSV_EINVAL, where + "unknown op kind " + std::to_string(o.kind));
+      default: return bad(TQSV_EINVAL, "unknown op kind " + std::to_string(o.kind));
     }
     if (g.param >= 0) {
-      if (g.cmask) return fail(TQSV_EUNSUPPORTED, where + "controlled ops cannot carry a parameter");
-      if ((g.gx | g.gz) & ~full) return fail(TQSV_EINVAL, where + "generator mask out of range");
-      if ((g.gx | g.gz) == 0) return fail(TQSV_EINVAL, where + "parametrised op without generator");
+      if (g.cmask) return bad(TQSV_EUNSUPPORTED, "controlled ops cannot carry a parameter");
+      if ((g.gx | g.gz) & ~full) return bad(TQSV_EINVAL, "generator mask out of range");
+      if ((g.gx | g.gz) == 0) return bad(TQSV_EINVAL, "parametrised op without generator");
     }
-    out.push_back(g);
   }
   return 0;
 }
 
-int pgrp_classes(const Gate& g, PgrpClass* out) {
+int pgrp_classes(const Gate& g, PgrpClass* out, bool constants) {
   int xb[64];
   int m = 0;
   for (uint64_t v = g.x; v; v &= v - 1) xb[m++] = ctz(v);
@@ -985,33 +987,42 @@ int pgrp_classes(const Gate& g, PgrpClass* out) {
     amax += std::abs(g.ma[r]);
     gmax += std::abs(g.mg[r]);
   }
-  // V(p) = sum_r w_r i^{k_r} (-1)^{|p & d_r|} for a pattern p of the x bits
-  auto V = [&](uint64_t p, const double* w) {
-    cd acc(0, 0);
-    for (int r = 0; r < g.nmem; ++r) {
-      const uint64_t d = (g.mz[r] ^ g.z) & g.x;
-      const int k = popc(g.x & g.mz[r]) & 3;
-      const double sg = (popc(p & d) & 1) ? -1.0 : 1.0;
-      acc += w[r] * sg * ipow[k];
-    }
-    return acc;
-  };
+  // V(p) = sum_r w_r i^{k_r} (-1)^{|p & d_r|} for a pattern p of the x bits,
+  // d_r = (z_r ^ z) & x.  Per member once: d_r, the phase-weighted angle and
+  // generator coefficient, and |d_r| mod 2, which relates the two members of
+  // a pair: V(p ^ x) = sum_r w_r i^{k_r} (-1)^{|p & d_r| + |d_r|}.
+  uint64_t dr[kMaxGroup];
+  cd wa[kMaxGroup], wg[kMaxGroup];
+  bool odd[kMaxGroup];
+  for (int r = 0; r < g.nmem; ++r) {
+    dr[r] = (g.mz[r] ^ g.z) & g.x;
+    const cd ph = ipow[popc(g.x & g.mz[r]) & 3];
+    wa[r] = g.ma[r] * ph;
+    wg[r] = g.mg[r] * ph;
+    odd[r] = (popc(dr[r]) & 1) != 0;
+  }
+  // one shared parameter: the classes that stay the identity for every
+  // value of it are known from the coefficients, so the set of active
+  // classes (part of the generated code) does not change with the angle
+  bool shared = g.nmem > 0 && g.mp[0] >= 0;
+  for (int r = 1; r < g.nmem; ++r) shared = shared && g.mp[r] == g.mp[0];
   for (int c = 0; c < nc; ++c) {
     uint64_t pi = 0;
     for (int b = 1; b < m; ++b)
       if ((c >> (b - 1)) & 1) pi |= bitm(xb[b]);
-    const uint64_t pj = pi ^ g.x;
-    const cd vi = V(pi, g.ma), vj = V(pj, g.ma);
+    cd vi(0, 0), vj(0, 0), vg(0, 0);
+    for (int r = 0; r < g.nmem; ++r) {
+      const bool ni = (popc(pi & dr[r]) & 1) != 0;
+      vi += ni ? -wa[r] : wa[r];
+      vj += (ni != odd[r]) ? -wa[r] : wa[r];
+      vg += ni ? -wg[r] : wg[r];
+    }
     const double w = 0.5 * (std::abs(vi) + std::abs(vj));
     PgrpClass& k = out[c];
     bool act = w > 1e-15 * std::max(amax, 1e-300);
-    // one shared parameter: the classes that stay the identity for every
-    // value of it are known from the coefficients, so the set of active
-    // classes (part of the generated code) does not change with the angle
-    bool shared = g.nmem > 0 && g.mp[0] >= 0;
-    for (int r = 1; r < g.nmem; ++r) shared = shared && g.mp[r] == g.mp[0];
-    if (shared && gmax > 0) act = act || std::abs(V(pi, g.mg)) > 1e-12 * gmax;
+    if (shared && gmax > 0) act = act || std::abs(vg) > 1e-12 * gmax;
     k.active = act;
+    if (!constants) continue;
     const double sq = std::sin(0.25 * w);
     k.cm1 = -2.0 * sq * sq;
     const double sow = w > 0 ? std::sin(0.5 * w) / w : 0.5;
@@ -1030,12 +1041,16 @@ int pgrp_classes(const Gate& g, PgrpClass* out) {
 // changes (1 in 8 for doubles) with one rounding, instead of every amplitude
 // once per string.
 void group_rotations(std::vector<Gate>& gs) {
-  std::vector<Gate> out;
-  out.reserve(gs.size());
+  // compacted in place: the write position w never passes the read position i
+  size_t w = 0;
+  auto put = [&](size_t i) {
+    if (w != i) gs[w] = gs[i];
+    ++w;
+  };
   for (size_t i = 0; i < gs.size(); ++i) {
     const Gate& g = gs[i];
     if (g.kind != GK_PROT || g.x == 0 || popc(g.x) > RB) {
-      out.push_back(g);
+      put(i);
       continue;
     }
     size_t j = i + 1;
@@ -1046,7 +1061,7 @@ void group_rotations(std::vector<Gate>& gs) {
       ++j;
     }
     if (j - i < 2) {
-      out.push_back(g);
+      put(i);
       continue;
     }
     Gate grp = g;
@@ -1061,10 +1076,10 @@ void group_rotations(std::vector<Gate>& gs) {
       grp.param = std::max(grp.param, gs[r].param);
     }
     classify(grp);
-    out.push_back(grp);
+    gs[w++] = grp;
     i = j - 1;
   }
-  gs.swap(out);
+  gs.resize(w);
 }
 
 void merge_and_classify(std::vector<Gate>& gs) {
@@ -1090,14 +1105,14 @@ void merge_and_classify(std::vector<Gate>& gs) {
     }
     for (uint64_t v = raw_used(g); v; v &= v - 1) last[ctz(v)] = (int)i;
   }
-  std::vector<Gate> out;
-  out.reserve(gs.size());
+  size_t w = 0;  // compacted in place
   for (size_t i = 0; i < gs.size(); ++i)
     if (!dead[i]) {
       classify(gs[i]);
-      out.push_back(gs[i]);
+      if (w != i) gs[w] = gs[i];
+      ++w;
     }
-  gs.swap(out);
+  gs.resize(w);
   group_rotations(gs);
 }
 

@@ -72,8 +72,9 @@ struct PgrpClass {
   double cm1 = 0.0;     // cos(w/2) - 1
   cd ui{0, 0}, uj{0, 0};
 };
-// Returns the number of classes (2^(|x|-1)).
-int pgrp_classes(const Gate& g, PgrpClass* out);
+// Returns the number of classes (2^(|x|-1)).  constants = false: only the
+// `active` flags (no trigonometry; for cost estimates).
+int pgrp_classes(const Gate& g, PgrpClass* out, bool constants = true);
 
 // Convert ABI ops into gates (validates).  Returns 0 or an error code.
 int gates_from_ops(int n, const tqsv_op* ops, int nops, std::vector<Gate>& out);
@@ -166,8 +167,18 @@ int jit_policy(int requested, int n, size_t tile_passes, size_t max_auto_passes)
 // and set PassPlan::jit_fn.
 int jit_prepare(std::vector<PassPlan>& passes, const std::vector<uint32_t>& prog, int n);
 hipError_t jit_launch(const PassPlan& pp, const PassLaunch& pl, hipStream_t st);
+// Compile the kernel of every tile pass without loading it (no device is
+// used; the compiler's own disk cache keeps the code objects for later
+// processes).  *compiled = distinct kernels compiled.
+int jit_compile_only(const std::vector<PassPlan>& passes, const std::vector<uint32_t>& prog, int n,
+                     int* compiled);
 // generated source of one pass record (for inspection and tests)
 std::string jit_source(int tb, const uint32_t* pass_words, int n);
+
+// The reverse sweep of a forward gate list (tqsv_ham.cpp): per gate in
+// reverse order, the gradient reduction(s) of its parameter(s), then its
+// inverse.
+void reverse_gates(const std::vector<Gate>& gates, std::vector<Gate>& r);
 
 }  // namespace tqsv
 
@@ -197,7 +208,18 @@ struct tqsv_plan {
   size_t dpartials_cap = 0;
   double* dred = nullptr;
   size_t dred_cap = 0;
+  // energy of tqsv_adjoint: the partial sums of <psi|H psi> and the result,
+  // apart from the reverse sweep's buffers so that the forward sweep can be
+  // queued before the reverse programs are built
+  double* denergy = nullptr;
+  size_t denergy_cap = 0;
   int max_param = -1;
+  // Scratch of the rebuilds after a parameter update, swapped with the live
+  // vectors above: their capacity is reused, so an update of a large circuit
+  // (thousands of gates) touches no fresh memory pages.
+  std::vector<tqsv::Gate> gwork, rwork;
+  std::vector<tqsv::PassPlan> pwork, rpwork;
+  std::vector<uint32_t> progwork, rprogwork;
   // States of fewer than kPadQubits qubits run as kPadQubits-qubit states whose
   // extra high index bits stay zero (no gate touches them), so that they use
   // the tile passes instead of one launch per gate: the caller's 2^nu
