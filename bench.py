@@ -18,12 +18,21 @@ FETCH_SIZE / WRITE_SIZE summary when one exists for this workload.
 `cpu_baseline` times the oracle (NumPy complex128, the reference's per-gate
 contraction) on a bounded sample of the same workload on the host cores.
 
-Other workloads: --workload adjoint (22-qubit RY.RZ + CX-ring ansatz, 528
-parameters, periodic Heisenberg chain: adjoint energy+gradient vs the plain
-per-parameter GPU path), --workload hea (12-qubit hardware-efficient ansatz,
-open-chain TFIM energy+gradient).
+The default line also carries the other two clauses of the metric as extra
+objects: `adjoint_22q` (the 22-qubit adjoint energy+gradient rate against the
+plain per-parameter GPU path, its error against the stored complex128 result
+and the dual pass's roofline) and `serve` (the continuous-batching server
+against the one-at-a-time loop at one offered rate); --no-extras leaves them
+out.
 
-    python bench.py [--steps K] [--warmup W] [--workload brickwork|adjoint|hea]
+Other workloads: --workload adjoint (22-qubit RY.RZ + CX-ring ansatz, 528
+parameters, periodic Heisenberg chain), hea (12-qubit hardware-efficient
+ansatz, open-chain TFIM energy+gradient), ucc (UCCSD, 20 spin-orbitals, on
+generated kernels), serve (per-rate table of the server against the plain
+loop), sample (10^6 shots from a 30-qubit state), mfma (a dense 5-qubit block
+through the vendor FP32 matrix-core GEMM against the fused VALU pass).
+
+    python bench.py [--steps K] [--warmup W] [--workload brickwork|adjoint|hea|ucc|serve|sample|mfma]
 """
 from __future__ import annotations
 
