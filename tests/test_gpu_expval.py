@@ -75,6 +75,64 @@ def test_ham_apply(gpu, n):
     np.testing.assert_allclose(to_np(out), want, atol=1e-5)
 
 
+def _shared_chain_terms(rng, n, groups):
+    """Terms that share a flip mask and the Z string outside it (the structure
+    of Jordan-Wigner strings): per group 2-6 flip qubits, a common Z chain
+    outside them and several X/Y/Z letter patterns on the flip qubits."""
+    terms = []
+    for _ in range(groups):
+        w = int(rng.integers(2, 7))
+        xq = [int(q) for q in rng.choice(n, w, replace=False)]
+        rest = [q for q in range(n) if q not in xq]
+        zq = [int(q) for q in rng.choice(rest, int(rng.integers(0, len(rest) + 1)), replace=False)]
+        for _ in range(int(rng.integers(2, 7))):
+            letters = ["XY"[int(rng.integers(2))] for _ in xq]
+            terms.append((float(rng.normal()), list(zip(letters, xq)) + [("Z", q) for q in zq]))
+    return terms
+
+
+@pytest.mark.parametrize("n", [10, 14, 21])
+def test_ham_apply_shared_chains_and_new_coefficients(gpu, n):
+    """H|psi> for terms that share flip masks and Z chains (the per-pattern
+    coefficient tables of the apply kernel), mixed with terms that share
+    nothing and diagonal terms, against the oracle; again after
+    set_coeffs, at sizes up to 21 qubits."""
+    import torch
+
+    rng = np.random.default_rng(100 + n)
+    v = rand_state(rng, n)
+    big = n > 18
+    terms = (_shared_chain_terms(rng, n, 4 if big else 12)
+             + random_pauli_terms(rng, n, 4 if big else 15, maxw=3))
+    terms += [(0.3, [("Z", 1)]), (-0.7, [("Z", 0), ("Z", n - 1)])]
+    h = _ham(n, terms)
+    out = torch.empty(1 << n, dtype=torch.complex64, device="cuda")
+    for rnd in range(2):
+        h.apply(to_gpu(v), out)
+        want = sum(c * O.pauli_apply(v, fac, n) for c, fac in terms)
+        np.testing.assert_allclose(to_np(out), want, atol=2e-5)
+        new = rng.normal(size=len(terms))
+        terms = [(float(c), fac) for c, (_, fac) in zip(new, terms)]
+        h.set_coeffs([c for c, _ in terms])
+
+
+def test_ham_apply_jordan_wigner_hamiltonian(gpu):
+    import torch
+
+    from tyxonq_amd import chem
+
+    ns = 5
+    n = 2 * ns
+    hh, gg = chem.synthetic_integrals(ns, 7)
+    terms = chem.molecular_hamiltonian(hh, gg)
+    v = rand_state(np.random.default_rng(3), n)
+    h = _ham(n, terms)
+    out = torch.empty(1 << n, dtype=torch.complex64, device="cuda")
+    h.apply(to_gpu(v), out)
+    want = sum(c * O.pauli_apply(v, fac, n) for c, fac in terms)
+    np.testing.assert_allclose(to_np(out), want, atol=2e-5)
+
+
 def test_expval_deterministic(gpu):
     n = 18
     rng = np.random.default_rng(1)

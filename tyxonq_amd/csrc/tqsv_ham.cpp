@@ -52,6 +52,19 @@ struct tqsv_ham {
   uint64_t* d_tz = nullptr;
   float2* d_tc = nullptr;
   std::vector<int> apply_order;  // term of each H|psi> coefficient slot
+  // the same H in the two-part form of k_ham_apply (HamApply): table
+  // subgroups (n <= 32) and the remaining terms as flip groups
+  std::vector<uint32_t> sub;               // 4 words per subgroup
+  std::vector<std::vector<int>> sub_terms;  // terms of each subgroup
+  uint32_t* d_sub = nullptr;
+  float2* d_stab = nullptr;
+  size_t stab_size = 0;
+  uint32_t nrest = 0;
+  uint64_t* d_rgx = nullptr;
+  uint32_t* d_rgstart = nullptr;
+  uint64_t* d_rtz = nullptr;
+  float2* d_rtc = nullptr;
+  std::vector<int> rest_order;
   bool jit_ready = false;        // generated expectation kernels compiled
 
   ~tqsv_ham() {
@@ -64,6 +77,24 @@ struct tqsv_ham {
     (void)hipFree(d_gstart);
     (void)hipFree(d_tz);
     (void)hipFree(d_tc);
+    (void)hipFree(d_sub);
+    (void)hipFree(d_stab);
+    (void)hipFree(d_rgx);
+    (void)hipFree(d_rgstart);
+    (void)hipFree(d_rtz);
+    (void)hipFree(d_rtc);
+  }
+  tqsv::HamApply apply_args() const {
+    tqsv::HamApply a;
+    a.nsub = (uint32_t)(sub.size() / 4);
+    a.sub = d_sub;
+    a.tab = d_stab;
+    a.ngroups = nrest;
+    a.gx = d_rgx;
+    a.gstart = d_rgstart;
+    a.tz = d_rtz;
+    a.tc = d_rtc;
+    return a;
   }
 };
 
@@ -307,11 +338,11 @@ int upload_expval(tqsv_ham* h) {
   return 0;
 }
 
-// coefficients of H|psi> in group order: coeff * i^k
-std::vector<float2> apply_coeffs(const tqsv_ham* h) {
+// coefficients of H|psi> in group order (of `order`): coeff * i^k
+std::vector<float2> apply_coeffs(const tqsv_ham* h, const std::vector<int>& order) {
   std::vector<float2> tc;
-  tc.reserve(h->apply_order.size());
-  for (int t : h->apply_order) {
+  tc.reserve(order.size());
+  for (int t : order) {
     const int k = popc(h->x[t] & h->z[t]) & 3;
     const double c = h->coeff[t];
     float2 v;
@@ -324,6 +355,31 @@ std::vector<float2> apply_coeffs(const tqsv_ham* h) {
     tc.push_back(v);
   }
   return tc;
+}
+
+// the coefficient tables of the table subgroups, formed in FP64: entry q of a
+// subgroup with flip mask x = sum_t c_t i^{k_t} (-1)^{|dep_x(q) & z_t|}
+std::vector<float2> sub_tables(const tqsv_ham* h) {
+  static const std::complex<double> ipow[4] = {{1, 0}, {0, 1}, {-1, 0}, {0, -1}};
+  std::vector<float2> tab;
+  for (size_t g = 0; g < h->sub_terms.size(); ++g) {
+    const uint32_t x = h->sub[4 * g];
+    int pos[32];
+    int m = 0;
+    for (uint32_t v = x; v; v &= v - 1) pos[m++] = ctz(v);
+    for (uint32_t q = 0; q < (1u << m); ++q) {
+      uint64_t dep = 0;
+      for (int k = 0; k < m; ++k)
+        if ((q >> k) & 1u) dep |= 1ull << pos[k];
+      std::complex<double> c(0, 0);
+      for (int t : h->sub_terms[g]) {
+        const double sg = (popc(dep & h->z[t]) & 1) ? -1.0 : 1.0;
+        c += sg * h->coeff[t] * ipow[popc(h->x[t] & h->z[t]) & 3];
+      }
+      tab.push_back(make_float2((float)c.real(), (float)c.imag()));
+    }
+  }
+  return tab;
 }
 
 int plan_apply(tqsv_ham* h) {
@@ -343,7 +399,53 @@ int plan_apply(tqsv_ham* h) {
   gstart.push_back((uint32_t)tzv.size());
   h->ngroups = (uint32_t)gx.size();
   if (upload(&h->d_gx, gx) || upload(&h->d_gstart, gstart) || upload(&h->d_tz, tzv) ||
-      upload(&h->d_tc, apply_coeffs(h)))
+      upload(&h->d_tc, apply_coeffs(h, h->apply_order)))
+    return TQSV_EDEVICE;
+  // two-part form: terms sharing x (1..6 bits) and the Z mask outside x,
+  // at least 2 of them, become a table subgroup; the rest stay per term
+  constexpr int kTabBits = 6;
+  std::map<std::pair<uint64_t, uint64_t>, std::vector<int>> subs;
+  if (h->n <= 32)
+    for (int t = 0; t < h->nterms; ++t)
+      if (h->x[t] && popc(h->x[t]) <= kTabBits) subs[{h->x[t], h->z[t] & ~h->x[t]}].push_back(t);
+  std::vector<char> in_table(h->nterms, 0);
+  h->sub.clear();
+  h->sub_terms.clear();
+  uint32_t off = 0;
+  for (auto& kv : subs) {
+    if (kv.second.size() < 2) continue;
+    const uint64_t x = kv.first.first;
+    // shifts: bit k of the pattern is bit k of j >> (position of the k-th
+    // flip bit - k); 5 bits each (at most 6 flip bits)
+    uint32_t pk = 0;
+    int k = 0;
+    for (uint64_t v = x; v; v &= v - 1, ++k) pk |= (uint32_t)(ctz(v) - k) << (5 * k);
+    h->sub.insert(h->sub.end(), {(uint32_t)x, (uint32_t)kv.first.second, off, pk});
+    h->sub_terms.push_back(kv.second);
+    for (int t : kv.second) in_table[t] = 1;
+    off += 1u << popc(x);
+  }
+  std::map<uint64_t, std::vector<int>> rest;
+  for (int t = 0; t < h->nterms; ++t)
+    if (!in_table[t]) rest[h->x[t]].push_back(t);
+  std::vector<uint64_t> rgx, rtz;
+  std::vector<uint32_t> rgstart;
+  h->rest_order.clear();
+  for (auto& kv : rest) {
+    rgx.push_back(kv.first);
+    rgstart.push_back((uint32_t)rtz.size());
+    for (int t : kv.second) {
+      rtz.push_back(h->z[t]);
+      h->rest_order.push_back(t);
+    }
+  }
+  rgstart.push_back((uint32_t)rtz.size());
+  h->nrest = (uint32_t)rgx.size();
+  const std::vector<float2> tab = sub_tables(h);
+  h->stab_size = tab.size();
+  if (upload(&h->d_sub, h->sub) || upload(&h->d_stab, tab) || upload(&h->d_rgx, rgx) ||
+      upload(&h->d_rgstart, rgstart) || upload(&h->d_rtz, rtz) ||
+      upload(&h->d_rtc, apply_coeffs(h, h->rest_order)))
     return TQSV_EDEVICE;
   return 0;
 }
@@ -559,12 +661,35 @@ int plan_build_adjoint(tqsv_plan* p, hipStream_t st) {
   p->rpasses.swap(passes);
   p->rprog.swap(prog);
   p->nred_adj = (int)red_total;
+  // every pass's block partials in one buffer, summed by one kernel after the
+  // sweep: one entry {offset, blocks, stride, index} per reduction slot
+  size_t pcap = 0;
+  p->hslots.assign(4 * (size_t)red_total, 0);
+  auto slot = [&](uint32_t r, size_t off, uint32_t nb, uint32_t stride, uint32_t k) {
+    uint32_t* d = p->hslots.data() + 4 * (size_t)r;
+    d[0] = (uint32_t)off;
+    d[1] = nb;
+    d[2] = stride;
+    d[3] = k;
+  };
+  for (PassPlan& pp : p->rpasses) {
+    if (pp.tile && pp.nred) {
+      pp.part_off = pcap;
+      for (uint32_t k = 0; k < pp.nred; ++k) slot(pp.red_base + k, pcap, pp.grid, pp.nred, k);
+      pcap += (size_t)pp.grid * pp.nred;
+    } else if (!pp.tile && p->rgates[pp.single].kind == GK_RED) {
+      const uint32_t g = reduce_grid(p->n);
+      pp.part_off = pcap;
+      for (uint32_t k = 0; k < 2; ++k) slot(pp.red_base + k, pcap, g, 2, k);
+      pcap += 2 * (size_t)g;
+    }
+  }
+  if (pcap >= (1ull << 32)) return fail(TQSV_EINVAL, "reverse sweep: too many reduction partials");
   rc = plan_upload(&p->drprog, &p->drprog_cap, p->rprog, st);
   if (rc) return rc;
-  size_t pcap = (size_t)reduce_grid(p->n) * 2;
-  for (const PassPlan& pp : p->rpasses)
-    if (pp.tile) pcap = std::max(pcap, (size_t)pp.grid * pp.nred);
-  rc = ensure(&p->dpartials, &p->dpartials_cap, pcap);
+  rc = plan_upload(&p->dslots, &p->dslots_cap, p->hslots, st);
+  if (rc) return rc;
+  rc = ensure(&p->dpartials, &p->dpartials_cap, std::max<size_t>(pcap, (size_t)reduce_grid(p->n) * 2));
   if (rc) return rc;
   rc = ensure(&p->dred, &p->dred_cap, (size_t)red_total + 2);
   if (rc) return rc;
@@ -603,7 +728,8 @@ static void collect_grad(const tqsv_plan* p, const double* hred, int nparams, do
 // reverse sweep + gradient assembly; scale = 1 for energies, 1/2 for VJPs
 static int reverse_and_collect(tqsv_plan* p, float2* psi, float2* lam, int nparams, double* grad,
                                double scale, double* hred, hipStream_t st) {
-  int rc = run_passes(p->rpasses, p->rgates, p->n, p->drprog, psi, lam, p->dpartials, p->dred, st);
+  int rc = run_passes(p->rpasses, p->rgates, p->n, p->drprog, psi, lam, p->dpartials, p->dred, st,
+                      p->dslots, (uint32_t)p->nred_adj);
   if (rc) return rc;
   const size_t nr = (size_t)p->nred_adj + 2;
   TQSV_HIP(hipMemcpyAsync(hred, p->dred, nr * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -699,9 +825,15 @@ int tqsv_ham_set_coeffs(tqsv_ham* h, int nterms, const double* coeff, void* stre
     if (nterms != h->nterms) return fail(TQSV_EINVAL, "tqsv_ham_set_coeffs: term count differs");
     for (int t = 0; t < nterms; ++t) h->coeff[t] = coeff[t];
     if (h->apply_order.empty()) return 0;
-    const std::vector<float2> tc = apply_coeffs(h);
+    const std::vector<float2> tc = apply_coeffs(h, h->apply_order);
+    const std::vector<float2> rtc = apply_coeffs(h, h->rest_order);
+    const std::vector<float2> tab = sub_tables(h);
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     TQSV_HIP(hipMemcpyAsync(h->d_tc, tc.data(), tc.size() * sizeof(float2), hipMemcpyHostToDevice, st));
+    if (!rtc.empty())
+      TQSV_HIP(hipMemcpyAsync(h->d_rtc, rtc.data(), rtc.size() * sizeof(float2), hipMemcpyHostToDevice, st));
+    if (!tab.empty())
+      TQSV_HIP(hipMemcpyAsync(h->d_stab, tab.data(), tab.size() * sizeof(float2), hipMemcpyHostToDevice, st));
     TQSV_HIP(hipStreamSynchronize(st));
     return 0;
   } catch (const std::bad_alloc&) {
@@ -729,8 +861,7 @@ int tqsv_ham_apply(tqsv_ham* h, const void* psi, void* out, void* stream) {
   if (!h || !psi || !out) return fail(TQSV_EINVAL, "tqsv_ham_apply: NULL argument");
   if (psi == out) return fail(TQSV_EINVAL, "tqsv_ham_apply: out must not alias psi");
   TQSV_HIP(k_ham_apply(reinterpret_cast<const float2*>(psi), reinterpret_cast<float2*>(out), h->n,
-                       h->ngroups, h->d_gx, h->d_gstart, h->d_tz, h->d_tc,
-                       reinterpret_cast<hipStream_t>(stream)));
+                       h->apply_args(), reinterpret_cast<hipStream_t>(stream)));
   return 0;
 }
 
@@ -786,7 +917,7 @@ int tqsv_adjoint(tqsv_plan* p, tqsv_ham* h, void* psi, void* lam, int nparams, d
     }
     rc = run_passes(p->passes, p->gates, p->n, p->dprog, a, nullptr, nullptr, nullptr, st);
     if (rc) return rc;
-    TQSV_HIP(k_ham_apply(a, b, h->n, h->ngroups, h->d_gx, h->d_gstart, h->d_tz, h->d_tc, st));
+    TQSV_HIP(k_ham_apply(a, b, h->n, h->apply_args(), st));
     // E = Re <psi|H psi>
     TQSV_HIP(k_pauli_dot(a, b, p->n, 0, 0, p->denergy, st));
     TQSV_HIP(k_final_sum(p->denergy, eg, 2, p->denergy + 2 * (size_t)eg, st));

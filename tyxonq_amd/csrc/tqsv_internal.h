@@ -65,6 +65,10 @@ hipError_t k_reduce_prob(const float2* psi, int n, int mode, int bit, double* pa
 // out[r] = sum_g partials[g * nred + r] (fixed order)
 hipError_t k_final_sum(const double* partials, uint32_t ngroups, uint32_t nred, double* out,
                        hipStream_t st);
+// one final sum per slot over a sweep-wide partials buffer (desc: 4 words per
+// slot: offset, blocks, stride, index)
+hipError_t k_final_sum_slots(const double* partials, const uint32_t* desc, uint32_t nslots,
+                             double* out, hipStream_t st);
 // partials[g*2 + {0,1}] = block sums of (-1)^{|i&z|} conj(u[i]) v[i^x]  (grid = reduce_grid(n))
 hipError_t k_pauli_dot(const float2* u, const float2* v, int n, uint64_t x, uint64_t z,
                        double* partials, hipStream_t st);
@@ -72,10 +76,26 @@ hipError_t k_pauli_dot(const float2* u, const float2* v, int n, uint64_t x, uint
 hipError_t k_pauli_dot_terms(const float2* u, const float2* v, int n, uint32_t nterms,
                              const uint64_t* dx, const uint64_t* dz, double* out2,
                              hipStream_t st);
-// out = sum_k coef_k P_k psi, terms given as device arrays
-hipError_t k_ham_apply(const float2* psi, float2* out, int n, uint32_t ngroups,
-                       const uint64_t* gx, const uint32_t* gstart, const uint64_t* tz,
-                       const float2* tc, hipStream_t st);
+// out = sum_k coef_k P_k psi, terms given as device arrays, in two parts.
+// Table part (n <= 32): subgroups of terms sharing the flip mask x and the Z
+// mask outside x, zc; 4 words each {x, zc, table offset, pattern shifts:
+// 5 bits each: position of the k-th flip bit (ascending) minus k, at most 6
+// flip bits}.  The table holds, per pattern q of the partner
+// index j = i ^ x on the bits of x, the summed coefficient
+// sum_t c_t i^{k_t} (-1)^{|dep(q) & z_t|}, so that a subgroup adds
+// (-1)^{|j & zc|} tab[off + q] psi[j].  Term part: flip groups x with their
+// terms' z masks and coefficients c_t i^{k_t}.
+struct HamApply {
+  uint32_t nsub = 0;
+  const uint32_t* sub = nullptr;
+  const float2* tab = nullptr;
+  uint32_t ngroups = 0;
+  const uint64_t* gx = nullptr;
+  const uint32_t* gstart = nullptr;
+  const uint64_t* tz = nullptr;
+  const float2* tc = nullptr;
+};
+hipError_t k_ham_apply(const float2* psi, float2* out, int n, const HamApply& H, hipStream_t st);
 
 // ---- batched energy+gradient of small requests (tqsv_batch.hip) ----
 // One work-group per request; every plan is built for BATCH_QUBITS qubits

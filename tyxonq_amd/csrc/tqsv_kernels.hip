@@ -195,6 +195,20 @@ __global__ __launch_bounds__(BLK) void final_sum_kernel(const double* partials, 
   if (threadIdx.x == 0) out[r] = s;
 }
 
+// one block per output slot r: the sum over the slot's blocks of partials
+// (entry r of `desc`: {offset, blocks, stride, index}: partials[offset + b *
+// stride + index] for b < blocks), in a fixed order
+__global__ __launch_bounds__(BLK) void final_sum_slots_kernel(const double* partials,
+                                                              const uint32_t* desc, double* out) {
+  const uint32_t r = blockIdx.x;
+  const uint32_t off = desc[4 * r], nb = desc[4 * r + 1], stride = desc[4 * r + 2],
+                 k = desc[4 * r + 3];
+  double acc = 0.0;
+  for (uint32_t g = threadIdx.x; g < nb; g += BLK) acc += partials[(size_t)off + (size_t)g * stride + k];
+  const double s = block_sum(acc);
+  if (threadIdx.x == 0) out[r] = s;
+}
+
 // block partial sums of (-1)^{|i&z|} conj(u[i]) v[i^x]
 __global__ __launch_bounds__(BLK) void pauli_dot_kernel(const float2* u, const float2* v, uint64_t dim,
                                                         uint64_t x, uint64_t z, double* partials) {
@@ -249,19 +263,71 @@ __global__ __launch_bounds__(BLK) void pauli_dot_terms_kernel(const float2* u, c
 
 // out[i] = sum over flip groups x of c_x(i) psi[i ^ x], c_x(j) = sum_t c_t i^k_t
 // (-1)^{|j & z_t|} over the group's terms; one thread per output amplitude.
-// The group and term arrays are read-only and indexed uniformly across the
-// wave, so they come through scalar loads; the term sign is an XOR of the
-// coefficient's sign bit with the parity of j & z_t (32-bit indices up to 32
-// qubits).
+// Terms that share x and the Z mask outside x (zc) come as one table
+// subgroup: c(j) = (-1)^{|j & zc|} tab[pattern of j on the bits of x], one
+// coefficient load instead of one parity and sign per term (the 14251 terms of
+// a 20-spin-orbital Jordan-Wigner Hamiltonian are 2445 such subgroups of 4
+// plus 102 groups).  Headers and term arrays are read-only and indexed
+// uniformly across the wave, so they come through scalar loads; signs are an
+// XOR of the sign bit with a parity (32-bit indices up to 32 qubits).
 template <typename IDX>
 __global__ void ham_apply_kernel(const float2* __restrict__ psi, float2* __restrict__ out,
-                                 uint64_t dim, uint32_t ngroups, const uint64_t* __restrict__ gx,
+                                 uint64_t dim, uint32_t nsub, const uint32_t* __restrict__ sub,
+                                 const float2* __restrict__ tab, uint32_t ngroups,
+                                 const uint64_t* __restrict__ gx,
                                  const uint32_t* __restrict__ gstart,
                                  const uint64_t* __restrict__ tz, const float2* __restrict__ tc) {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i0 < dim; i0 += stride) {
     const IDX i = (IDX)i0;
-    double ar = 0.0, ai = 0.0;  // FP64 across the groups, one rounding at the end
+    double ar = 0.0, ai = 0.0;  // FP64 across subgroups and groups, one rounding at the end
+    if constexpr (sizeof(IDX) == 4) {
+      // table part: per subgroup one partner load, the pattern of the partner
+      // on the flip bits (bit k of the pattern = bit k of j >> sh_k, sh_k =
+      // position of the k-th flip bit - k, packed 5 bits each), one
+      // coefficient load and the sign of the Z chain.  Four subgroups per
+      // round: all eight loads are issued before the arithmetic of the first,
+      // so their latencies overlap.
+      auto pattern = [](uint32_t j, uint32_t x, uint32_t pk) {
+        const uint32_t m = (uint32_t)__popc(x);
+        uint32_t q = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < 6; ++k)
+          if (k < m) q |= (j >> ((pk >> (5 * k)) & 31u)) & (1u << k);
+        return q;
+      };
+      uint32_t g = 0;
+      for (; g + 4 <= nsub; g += 4) {
+        uint32_t jj[4], zc[4];
+        float2 v[4], c[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const uint32_t* d = sub + 4 * (g + u);
+          jj[u] = (uint32_t)i ^ d[0];
+          zc[u] = d[1];
+          v[u] = psi[jj[u]];
+          c[u] = tab[d[2] + pattern(jj[u], d[0], d[3])];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const uint32_t neg = par(jj[u] & zc[u]) << 31;
+          const float re = c[u].x * v[u].x - c[u].y * v[u].y;
+          const float im = c[u].x * v[u].y + c[u].y * v[u].x;
+          ar += (double)__uint_as_float(__float_as_uint(re) ^ neg);
+          ai += (double)__uint_as_float(__float_as_uint(im) ^ neg);
+        }
+      }
+      for (; g < nsub; ++g) {
+        const uint32_t* d = sub + 4 * g;
+        const uint32_t j = (uint32_t)i ^ d[0];
+        const float2 v = psi[j];
+        const float2 c = tab[d[2] + pattern(j, d[0], d[3])];
+        const uint32_t neg = par(j & d[1]) << 31;
+        const float re = c.x * v.x - c.y * v.y, im = c.x * v.y + c.y * v.x;
+        ar += (double)__uint_as_float(__float_as_uint(re) ^ neg);
+        ai += (double)__uint_as_float(__float_as_uint(im) ^ neg);
+      }
+    }
     for (uint32_t g = 0; g < ngroups; ++g) {
       const IDX j = i ^ (IDX)gx[g];
       const float2 v = psi[j];
@@ -421,6 +487,13 @@ hipError_t k_final_sum(const double* partials, uint32_t ngroups, uint32_t nred, 
   return hipGetLastError();
 }
 
+hipError_t k_final_sum_slots(const double* partials, const uint32_t* desc, uint32_t nslots,
+                             double* out, hipStream_t st) {
+  if (nslots == 0) return hipSuccess;
+  hipLaunchKernelGGL(final_sum_slots_kernel, dim3(nslots), dim3(BLK), 0, st, partials, desc, out);
+  return hipGetLastError();
+}
+
 hipError_t k_pauli_dot(const float2* u, const float2* v, int n, uint64_t x, uint64_t z,
                        double* partials, hipStream_t st) {
   const uint64_t dim = 1ull << n;
@@ -439,16 +512,14 @@ hipError_t k_pauli_dot_terms(const float2* u, const float2* v, int n, uint32_t n
   return hipGetLastError();
 }
 
-hipError_t k_ham_apply(const float2* psi, float2* out, int n, uint32_t ngroups, const uint64_t* gx,
-                       const uint32_t* gstart, const uint64_t* tz, const float2* tc,
-                       hipStream_t st) {
+hipError_t k_ham_apply(const float2* psi, float2* out, int n, const HamApply& H, hipStream_t st) {
   const uint64_t dim = 1ull << n;
   if (n <= 32)
     hipLaunchKernelGGL(ham_apply_kernel<uint32_t>, dim3(grid_for(dim)), dim3(BLK), 0, st, psi, out,
-                       dim, ngroups, gx, gstart, tz, tc);
+                       dim, H.nsub, H.sub, H.tab, H.ngroups, H.gx, H.gstart, H.tz, H.tc);
   else
     hipLaunchKernelGGL(ham_apply_kernel<uint64_t>, dim3(grid_for(dim)), dim3(BLK), 0, st, psi, out,
-                       dim, ngroups, gx, gstart, tz, tc);
+                       dim, 0u, nullptr, nullptr, H.ngroups, H.gx, H.gstart, H.tz, H.tc);
   return hipGetLastError();
 }
 

@@ -1509,14 +1509,16 @@ static hipError_t apply_single(const Gate& g, int n, float2* psi, hipStream_t st
 
 int run_passes(const std::vector<PassPlan>& passes, const std::vector<Gate>& gates, int n,
                const uint32_t* dprog, float2* psi, float2* lam, double* dpartials,
-               double* dred_out, hipStream_t st) {
+               double* dred_out, hipStream_t st, const uint32_t* dslots, uint32_t nslots) {
+  const bool one_sum = dslots != nullptr;
   for (const PassPlan& pp : passes) {
+    double* part = one_sum ? dpartials + pp.part_off : dpartials;
     if (pp.tile) {
       PassLaunch pl;
       pl.psi = psi;
       pl.lam = lam;
       pl.prog = dprog + pp.prog_off;
-      pl.partials = dpartials;
+      pl.partials = part;
       pl.ntiles = pp.ntiles;
       pl.grid = pp.grid;
       pl.lds_bytes = pp.lds;
@@ -1524,19 +1526,21 @@ int run_passes(const std::vector<PassPlan>& passes, const std::vector<Gate>& gat
         TQSV_HIP(jit_launch(pp, pl, st));
       else
         TQSV_HIP(launch_pass(pp.tb, pp.mode, n > 32, pp.lean, pl, st));
-      if (pp.nred) TQSV_HIP(k_final_sum(dpartials, pp.grid, pp.nred, dred_out + pp.red_base, st));
+      if (pp.nred && !one_sum)
+        TQSV_HIP(k_final_sum(dpartials, pp.grid, pp.nred, dred_out + pp.red_base, st));
     } else {
       const Gate& g = gates[pp.single];
       if (g.kind == GK_RED) {
         const uint32_t grid = reduce_grid(n);
-        TQSV_HIP(k_pauli_dot(lam, psi, n, g.x, g.z, dpartials, st));
-        TQSV_HIP(k_final_sum(dpartials, grid, 2, dred_out + pp.red_base, st));
+        TQSV_HIP(k_pauli_dot(lam, psi, n, g.x, g.z, part, st));
+        if (!one_sum) TQSV_HIP(k_final_sum(dpartials, grid, 2, dred_out + pp.red_base, st));
       } else {
         TQSV_HIP(apply_single(g, n, psi, st));
         if (pp.dual) TQSV_HIP(apply_single(g, n, lam, st));
       }
     }
   }
+  if (one_sum) TQSV_HIP(k_final_sum_slots(dpartials, dslots, nslots, dred_out, st));
   return 0;
 }
 

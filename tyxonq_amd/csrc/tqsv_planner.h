@@ -95,6 +95,7 @@ struct PassPlan {
   int single = -1;
   uint32_t nred = 0;      // reduction slots of this pass
   uint32_t red_base = 0;  // first slot in the plan's reduction output
+  size_t part_off = 0;    // this pass's block partials in a sweep-wide partials buffer
   // emitted program
   size_t prog_off = 0;   // words
   size_t prog_words = 0;
@@ -152,9 +153,15 @@ int emit_passes(std::vector<Gate>& gates, const SchedOpts& so, std::vector<PassP
 bool same_structure(const std::vector<Gate>& a, const std::vector<Gate>& b);
 
 // Run planned passes (device program at dprog; partial-sum scratch for reductions).
+// Reductions: without `dslots` each pass's block partials go to the start of
+// `dpartials` and are summed into its slots right after the pass; with
+// `dslots` (nslots entries of 4 words: partials offset, blocks, stride, index)
+// each pass writes at its part_off and one kernel after the sweep sums every
+// slot.
 int run_passes(const std::vector<PassPlan>& passes, const std::vector<Gate>& gates, int n,
                const uint32_t* dprog, float2* psi, float2* lam, double* dpartials,
-               double* dred_out, hipStream_t st);
+               double* dred_out, hipStream_t st, const uint32_t* dslots = nullptr,
+               uint32_t nslots = 0);
 
 int effective_tile_bits(int n, int requested, bool dual);
 
@@ -213,6 +220,11 @@ struct tqsv_plan {
   // queued before the reverse programs are built
   double* denergy = nullptr;
   size_t denergy_cap = 0;
+  // per reduction slot of the reverse sweep: {partials offset, blocks, stride,
+  // index} (one final sum for the whole sweep)
+  std::vector<uint32_t> hslots;
+  uint32_t* dslots = nullptr;
+  size_t dslots_cap = 0;
   int max_param = -1;
   // Scratch of the rebuilds after a parameter update, swapped with the live
   // vectors above: their capacity is reused, so an update of a large circuit
