@@ -559,8 +559,10 @@ def run_ucc(args) -> dict:
                    "max_abs_dgrad": float(np.max(np.abs(g0 - np.asarray(gold["grad"])))),
                    "params_compared": len(gold["grad"])}
 
+    opbuf = ans.ops(thetas[0])
+
     def step(th):
-        plan.update(ans.ops(th))
+        plan.update(ans.ops(th, out=opbuf))
         nat.init_basis(psi, 0)
         return plan.adjoint(ham, psi, lam)
 

@@ -292,8 +292,12 @@ class UCCAnsatz:
     def nrot(self) -> int:
         return len(self.pid)
 
-    def ops(self, params) -> np.ndarray:
+    def ops(self, params, out: np.ndarray | None = None) -> np.ndarray:
+        """The op array for `params`.  `out`: an array returned by an earlier
+        call, refilled in place (only the rotation angles change; a fresh
+        2 MB copy per parameter update costs more than the update itself)."""
         th = np.asarray(params, dtype=np.float64)
-        out = self._ops.copy()
+        if out is None or out.shape != self._ops.shape or out.dtype != self._ops.dtype:
+            out = self._ops.copy()
         out["theta"][self.hf_rows:] = self.coef * th[self.pid]
         return out

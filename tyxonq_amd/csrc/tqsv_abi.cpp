@@ -439,7 +439,6 @@ void tqsv_plan_destroy(tqsv_plan* p) {
   (void)hipFree(p->drprog);
   (void)hipFree(p->dpartials);
   (void)hipFree(p->dred);
-  (void)hipFree(p->denergy);
   (void)hipFree(p->dslots);
   (void)hipFree(p->pad_a);
   (void)hipFree(p->pad_b);

@@ -401,9 +401,9 @@ int plan_apply(tqsv_ham* h) {
   if (upload(&h->d_gx, gx) || upload(&h->d_gstart, gstart) || upload(&h->d_tz, tzv) ||
       upload(&h->d_tc, apply_coeffs(h, h->apply_order)))
     return TQSV_EDEVICE;
-  // two-part form: terms sharing x (1..6 bits) and the Z mask outside x,
+  // two-part form: terms sharing x (1..4 bits) and the Z mask outside x,
   // at least 2 of them, become a table subgroup; the rest stay per term
-  constexpr int kTabBits = 6;
+  constexpr int kTabBits = 4;
   std::map<std::pair<uint64_t, uint64_t>, std::vector<int>> subs;
   if (h->n <= 32)
     for (int t = 0; t < h->nterms; ++t)
@@ -416,10 +416,12 @@ int plan_apply(tqsv_ham* h) {
     if (kv.second.size() < 2) continue;
     const uint64_t x = kv.first.first;
     // shifts: bit k of the pattern is bit k of j >> (position of the k-th
-    // flip bit - k); 5 bits each (at most 6 flip bits)
+    // flip bit - k); 5 bits each, 31 for unused positions (bit k >= 1 of
+    // j >> 31 is 0)
     uint32_t pk = 0;
     int k = 0;
     for (uint64_t v = x; v; v &= v - 1, ++k) pk |= (uint32_t)(ctz(v) - k) << (5 * k);
+    for (; k < kTabBits; ++k) pk |= 31u << (5 * k);
     h->sub.insert(h->sub.end(), {(uint32_t)x, (uint32_t)kv.first.second, off, pk});
     h->sub_terms.push_back(kv.second);
     for (int t : kv.second) in_table[t] = 1;
@@ -571,6 +573,27 @@ int plan_build_forward(tqsv_plan* p, const tqsv_op* ops, int nops, hipStream_t s
   return plan_upload(&p->dprog, &p->dprog_cap, p->prog, st);
 }
 
+// grow *d to `need` doubles keeping its first `keep`: work queued on st may
+// still write that region, so the stream is drained first
+static int grow_keep(double** d, size_t* cap, size_t need, size_t keep, hipStream_t st) {
+  need = std::max<size_t>(need, 1);
+  if (*cap >= need) return 0;
+  TQSV_HIP(hipStreamSynchronize(st));
+  double* nd = nullptr;
+  TQSV_HIP(hipMalloc(reinterpret_cast<void**>(&nd), need * sizeof(double)));
+  if (*d && keep) {
+    const hipError_t e = hipMemcpy(nd, *d, std::min(keep, *cap) * sizeof(double), hipMemcpyDeviceToDevice);
+    if (e != hipSuccess) {
+      (void)hipFree(nd);
+      return hip_fail(e, "grow_keep");
+    }
+  }
+  if (*d) TQSV_HIP(hipFree(*d));
+  *d = nd;
+  *cap = need;
+  return 0;
+}
+
 static int ensure(double** d, size_t* cap, size_t need) {
   need = std::max<size_t>(need, 1);
   if (*cap >= need) return 0;
@@ -662,9 +685,11 @@ int plan_build_adjoint(tqsv_plan* p, hipStream_t st) {
   p->rprog.swap(prog);
   p->nred_adj = (int)red_total;
   // every pass's block partials in one buffer, summed by one kernel after the
-  // sweep: one entry {offset, blocks, stride, index} per reduction slot
-  size_t pcap = 0;
-  p->hslots.assign(4 * (size_t)red_total, 0);
+  // sweep: one entry {offset, blocks, stride, index} per reduction slot; the
+  // energy's partials (from H|psi>, tqsv_adjoint) first, its two entries last
+  const uint32_t eg = ham_apply_grid(p->nu);  // H|psi> runs on the caller's qubits
+  size_t pcap = 2 * (size_t)eg;
+  p->hslots.assign(4 * ((size_t)red_total + 2), 0);
   auto slot = [&](uint32_t r, size_t off, uint32_t nb, uint32_t stride, uint32_t k) {
     uint32_t* d = p->hslots.data() + 4 * (size_t)r;
     d[0] = (uint32_t)off;
@@ -684,12 +709,15 @@ int plan_build_adjoint(tqsv_plan* p, hipStream_t st) {
       pcap += 2 * (size_t)g;
     }
   }
+  slot(red_total, 0, eg, 2, 0);
+  slot(red_total + 1, 0, eg, 2, 1);
   if (pcap >= (1ull << 32)) return fail(TQSV_EINVAL, "reverse sweep: too many reduction partials");
   rc = plan_upload(&p->drprog, &p->drprog_cap, p->rprog, st);
   if (rc) return rc;
   rc = plan_upload(&p->dslots, &p->dslots_cap, p->hslots, st);
   if (rc) return rc;
-  rc = ensure(&p->dpartials, &p->dpartials_cap, std::max<size_t>(pcap, (size_t)reduce_grid(p->n) * 2));
+  // the energy partials of a queued H|psi> may already be in the buffer
+  rc = grow_keep(&p->dpartials, &p->dpartials_cap, pcap, 2 * (size_t)eg, st);
   if (rc) return rc;
   rc = ensure(&p->dred, &p->dred_cap, (size_t)red_total + 2);
   if (rc) return rc;
@@ -727,9 +755,11 @@ static void collect_grad(const tqsv_plan* p, const double* hred, int nparams, do
 
 // reverse sweep + gradient assembly; scale = 1 for energies, 1/2 for VJPs
 static int reverse_and_collect(tqsv_plan* p, float2* psi, float2* lam, int nparams, double* grad,
-                               double scale, double* hred, hipStream_t st) {
+                               double scale, double* hred, hipStream_t st, bool energy) {
+  // with `energy` the final sum also forms the energy slots (after the
+  // reverse sweep's) from the partials of the queued H|psi>
   int rc = run_passes(p->rpasses, p->rgates, p->n, p->drprog, psi, lam, p->dpartials, p->dred, st,
-                      p->dslots, (uint32_t)p->nred_adj);
+                      p->dslots, (uint32_t)p->nred_adj + (energy ? 2u : 0u));
   if (rc) return rc;
   const size_t nr = (size_t)p->nred_adj + 2;
   TQSV_HIP(hipMemcpyAsync(hred, p->dred, nr * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -903,8 +933,10 @@ int tqsv_adjoint(tqsv_plan* p, tqsv_ham* h, void* psi, void* lam, int nparams, d
       return fail(TQSV_EINVAL, "tqsv_adjoint: nparams smaller than the largest parameter index + 1");
     if (psi == lam) return fail(TQSV_EINVAL, "tqsv_adjoint: lam must not alias psi");
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    const uint32_t eg = reduce_grid(p->n);
-    int rc = ensure(&p->denergy, &p->denergy_cap, 2 * (size_t)eg + 2);
+    // the energy partials lead the plan's partials buffer (nothing is queued
+    // on the stream that still uses it: every sweep ends synchronised)
+    const uint32_t eg = ham_apply_grid(h->n);
+    int rc = grow_keep(&p->dpartials, &p->dpartials_cap, 2 * (size_t)eg, 0, st);
     if (rc) return rc;
     const bool pad = p->n > p->nu;
     float2* a = pad ? p->pad_a : reinterpret_cast<float2*>(psi);
@@ -917,19 +949,14 @@ int tqsv_adjoint(tqsv_plan* p, tqsv_ham* h, void* psi, void* lam, int nparams, d
     }
     rc = run_passes(p->passes, p->gates, p->n, p->dprog, a, nullptr, nullptr, nullptr, st);
     if (rc) return rc;
-    TQSV_HIP(k_ham_apply(a, b, h->n, h->apply_args(), st));
-    // E = Re <psi|H psi>
-    TQSV_HIP(k_pauli_dot(a, b, p->n, 0, 0, p->denergy, st));
-    TQSV_HIP(k_final_sum(p->denergy, eg, 2, p->denergy + 2 * (size_t)eg, st));
+    // lambda = H psi and the block partials of E = Re <psi|H psi>
+    TQSV_HIP(k_ham_apply(a, b, h->n, h->apply_args(), st, p->dpartials));
     // the reverse programs are built on the host while the device runs the
     // forward sweep and H|psi> (after a parameter update they are re-emitted)
     rc = plan_build_adjoint(p, st);
     if (rc) return rc;
-    // the energy joins the reverse sweep's slots (after them), read back with them
-    TQSV_HIP(hipMemcpyAsync(p->dred + p->nred_adj, p->denergy + 2 * (size_t)eg, 2 * sizeof(double),
-                            hipMemcpyDeviceToDevice, st));
     std::vector<double> hred((size_t)p->nred_adj + 2);
-    rc = reverse_and_collect(p, a, b, nparams, grad, 1.0, hred.data(), st);
+    rc = reverse_and_collect(p, a, b, nparams, grad, 1.0, hred.data(), st, true);
     if (rc) return rc;
     if (pad) {
       const size_t bytes = sizeof(float2) << p->nu;
@@ -1052,11 +1079,11 @@ int tqsv_vjp(tqsv_plan* p, void* psi, void* lam, int nparams, double* grad, void
     std::vector<double> hred((size_t)p->nred_adj + 2);
     if (p->n == p->nu)
       return reverse_and_collect(p, reinterpret_cast<float2*>(psi), reinterpret_cast<float2*>(lam),
-                                 nparams, grad, 0.5, hred.data(), st);
+                                 nparams, grad, 0.5, hred.data(), st, false);
     const uint64_t du = 1ull << p->nu, dn = 1ull << p->n;
     TQSV_HIP(k_pad_copy(reinterpret_cast<const float2*>(psi), du, p->pad_a, dn, st));
     TQSV_HIP(k_pad_copy(reinterpret_cast<const float2*>(lam), du, p->pad_b, dn, st));
-    rc = reverse_and_collect(p, p->pad_a, p->pad_b, nparams, grad, 0.5, hred.data(), st);
+    rc = reverse_and_collect(p, p->pad_a, p->pad_b, nparams, grad, 0.5, hred.data(), st, false);
     if (rc) return rc;
     TQSV_HIP(hipMemcpyAsync(psi, p->pad_a, du * sizeof(float2), hipMemcpyDeviceToDevice, st));
     TQSV_HIP(hipMemcpyAsync(lam, p->pad_b, du * sizeof(float2), hipMemcpyDeviceToDevice, st));

@@ -79,8 +79,8 @@ hipError_t k_pauli_dot_terms(const float2* u, const float2* v, int n, uint32_t n
 // out = sum_k coef_k P_k psi, terms given as device arrays, in two parts.
 // Table part (n <= 32): subgroups of terms sharing the flip mask x and the Z
 // mask outside x, zc; 4 words each {x, zc, table offset, pattern shifts:
-// 5 bits each: position of the k-th flip bit (ascending) minus k, at most 6
-// flip bits}.  The table holds, per pattern q of the partner
+// 5 bits each: position of the k-th flip bit (ascending) minus k, at most 4
+// flip bits, 31 for unused positions}.  The table holds, per pattern q of the partner
 // index j = i ^ x on the bits of x, the summed coefficient
 // sum_t c_t i^{k_t} (-1)^{|dep(q) & z_t|}, so that a subgroup adds
 // (-1)^{|j & zc|} tab[off + q] psi[j].  Term part: flip groups x with their
@@ -95,7 +95,11 @@ struct HamApply {
   const uint64_t* tz = nullptr;
   const float2* tc = nullptr;
 };
-hipError_t k_ham_apply(const float2* psi, float2* out, int n, const HamApply& H, hipStream_t st);
+// epart (optional): per block {Re, Im} of sum_i conj(psi_i) (H psi)_i, FP64
+// from the unrounded (H psi)_i; ham_apply_grid(n) blocks.
+hipError_t k_ham_apply(const float2* psi, float2* out, int n, const HamApply& H, hipStream_t st,
+                       double* epart = nullptr);
+uint32_t ham_apply_grid(int n);
 
 // ---- batched energy+gradient of small requests (tqsv_batch.hip) ----
 // One work-group per request; every plan is built for BATCH_QUBITS qubits

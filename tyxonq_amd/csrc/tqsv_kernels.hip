@@ -276,25 +276,32 @@ __global__ void ham_apply_kernel(const float2* __restrict__ psi, float2* __restr
                                  const float2* __restrict__ tab, uint32_t ngroups,
                                  const uint64_t* __restrict__ gx,
                                  const uint32_t* __restrict__ gstart,
-                                 const uint64_t* __restrict__ tz, const float2* __restrict__ tc) {
+                                 const uint64_t* __restrict__ tz, const float2* __restrict__ tc,
+                                 double* __restrict__ epart) {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i0 < dim; i0 += stride) {
+  double er = 0.0, ei = 0.0;  // <psi|H psi> of this thread's amplitudes (with epart)
+  // Work-groups are dispatched round-robin over the 8 XCDs (block id mod 8).
+  // Block b takes the b/8-th block of the (b mod 8)-th eighth of the index
+  // range, so each XCD works on one eighth.  The subgroups and groups come in
+  // ascending x, so all of an XCD's work-groups read partners from one eighth
+  // of the state at a time (j = i ^ x, top bits of x varying slowest), which
+  // its L2 can hold for states up to 2^22 amplitudes.
+  uint64_t b = blockIdx.x;
+  if ((gridDim.x & 7u) == 0) b = (b & 7u) * (gridDim.x >> 3) + (b >> 3);
+  for (uint64_t i0 = b * blockDim.x + threadIdx.x; i0 < dim; i0 += stride) {
     const IDX i = (IDX)i0;
     double ar = 0.0, ai = 0.0;  // FP64 across subgroups and groups, one rounding at the end
     if constexpr (sizeof(IDX) == 4) {
       // table part: per subgroup one partner load, the pattern of the partner
-      // on the flip bits (bit k of the pattern = bit k of j >> sh_k, sh_k =
-      // position of the k-th flip bit - k, packed 5 bits each), one
-      // coefficient load and the sign of the Z chain.  Four subgroups per
+      // on the (at most 4) flip bits: bit k = bit k of j >> sh_k, sh_k =
+      // position of the k-th flip bit - k, packed 5 bits each; unused
+      // positions hold 31, whose bit k >= 1 is always 0 for 32-bit indices),
+      // one coefficient load and the sign of the Z chain.  Four subgroups per
       // round: all eight loads are issued before the arithmetic of the first,
       // so their latencies overlap.
-      auto pattern = [](uint32_t j, uint32_t x, uint32_t pk) {
-        const uint32_t m = (uint32_t)__popc(x);
-        uint32_t q = 0;
-#pragma unroll
-        for (uint32_t k = 0; k < 6; ++k)
-          if (k < m) q |= (j >> ((pk >> (5 * k)) & 31u)) & (1u << k);
-        return q;
+      auto pattern = [](uint32_t j, uint32_t, uint32_t pk) {
+        return ((j >> (pk & 31u)) & 1u) | ((j >> ((pk >> 5) & 31u)) & 2u) |
+               ((j >> ((pk >> 10) & 31u)) & 4u) | ((j >> ((pk >> 15) & 31u)) & 8u);
       };
       uint32_t g = 0;
       for (; g + 4 <= nsub; g += 4) {
@@ -344,6 +351,21 @@ __global__ void ham_apply_kernel(const float2* __restrict__ psi, float2* __restr
       ai += (double)(fr * v.y) + (double)(fi * v.x);
     }
     out[i0] = make_float2((float)ar, (float)ai);
+    if (epart) {
+      // the energy from the unrounded (H psi)_i
+      const float2 p = psi[i];
+      er += (double)p.x * ar + (double)p.y * ai;
+      ei += (double)p.x * ai - (double)p.y * ar;
+    }
+  }
+  if (epart) {
+    const double sr = block_sum(er);
+    __syncthreads();
+    const double si = block_sum(ei);
+    if (threadIdx.x == 0) {
+      epart[2 * blockIdx.x] = sr;
+      epart[2 * blockIdx.x + 1] = si;
+    }
   }
 }
 
@@ -512,14 +534,17 @@ hipError_t k_pauli_dot_terms(const float2* u, const float2* v, int n, uint32_t n
   return hipGetLastError();
 }
 
-hipError_t k_ham_apply(const float2* psi, float2* out, int n, const HamApply& H, hipStream_t st) {
+uint32_t ham_apply_grid(int n) { return grid_for(1ull << n); }
+
+hipError_t k_ham_apply(const float2* psi, float2* out, int n, const HamApply& H, hipStream_t st,
+                       double* epart) {
   const uint64_t dim = 1ull << n;
   if (n <= 32)
     hipLaunchKernelGGL(ham_apply_kernel<uint32_t>, dim3(grid_for(dim)), dim3(BLK), 0, st, psi, out,
-                       dim, H.nsub, H.sub, H.tab, H.ngroups, H.gx, H.gstart, H.tz, H.tc);
+                       dim, H.nsub, H.sub, H.tab, H.ngroups, H.gx, H.gstart, H.tz, H.tc, epart);
   else
     hipLaunchKernelGGL(ham_apply_kernel<uint64_t>, dim3(grid_for(dim)), dim3(BLK), 0, st, psi, out,
-                       dim, 0u, nullptr, nullptr, H.ngroups, H.gx, H.gstart, H.tz, H.tc);
+                       dim, 0u, nullptr, nullptr, H.ngroups, H.gx, H.gstart, H.tz, H.tc, epart);
   return hipGetLastError();
 }
 

@@ -215,13 +215,10 @@ struct tqsv_plan {
   size_t dpartials_cap = 0;
   double* dred = nullptr;
   size_t dred_cap = 0;
-  // energy of tqsv_adjoint: the partial sums of <psi|H psi> and the result,
-  // apart from the reverse sweep's buffers so that the forward sweep can be
-  // queued before the reverse programs are built
-  double* denergy = nullptr;
-  size_t denergy_cap = 0;
   // per reduction slot of the reverse sweep: {partials offset, blocks, stride,
-  // index} (one final sum for the whole sweep)
+  // index} (one final sum for the whole sweep), then two entries for the
+  // energy of tqsv_adjoint, whose block partials (from H|psi>) lead the
+  // partials buffer
   std::vector<uint32_t> hslots;
   uint32_t* dslots = nullptr;
   size_t dslots_cap = 0;

@@ -125,6 +125,7 @@ class UCCNumericRuntime:
         self._ansatz = chem.UCCAnsatz(self.n_qubits, self.n_elec_s, self.ex_ops or [],
                                       self.param_ids or [])
         self._plan: Optional[nat.Plan] = None
+        self._opbuf: Optional[np.ndarray] = None  # op array refilled per parameter update
 
     def _params(self, params) -> np.ndarray:
         if params is None:
@@ -139,7 +140,7 @@ class UCCNumericRuntime:
     def _run(self, params):
         import torch
 
-        ops = self._ansatz.ops(self._params(params))
+        ops = self._opbuf = self._ansatz.ops(self._params(params), out=self._opbuf)
         if self._plan is None:
             self._plan = nat.Plan(self.n_qubits, ops, max_cost=100000)
         else:
