@@ -325,7 +325,7 @@ def run_adjoint(args, steps=None, warmup=None, n=None, layers=None) -> dict:
     L, _, _ = lower(circ, track_grad=True)
     ops = L.segments[0][1]
     t0 = time.perf_counter()
-    plan = nat.Plan(n, ops, max_cost=args.max_cost or 0, jit=args.jit)
+    plan = nat.Plan(n, ops, tile_bits=args.tile_bits or 0, max_cost=args.max_cost or 0, jit=args.jit)
     x, z, c = PauliSum(n, terms).to_masks()
     ham = nat.Ham(n, x, z, c)
     dev = nat.device()

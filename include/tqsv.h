@@ -244,6 +244,15 @@ typedef struct tqsv_ham tqsv_ham;
 int tqsv_ham_create(int n, int nterms, const uint64_t* xmask, const uint64_t* zmask,
                     const double* coeff, tqsv_ham** out);
 
+/* Plan a Pauli sum's expectation passes on the host and compile their
+ * generated kernels (the ones tqsv_ham_create would use: states of 2^22+
+ * amplitudes) into the compiler's disk cache, without a GPU; *kernels (may be
+ * NULL) = distinct kernels compiled (0 when the plan uses the precompiled
+ * kernel).  No reference counterpart; for start-up and tests, as
+ * tqsv_plan_precompile. */
+int tqsv_ham_precompile(int n, int nterms, const uint64_t* xmask, const uint64_t* zmask,
+                        const double* coeff, int* kernels);
+
 /* Per-term <psi|P_k|psi> into terms[nterms] (host, may be NULL), the energy
  * sum_k coeff_k <P_k> into *energy (host, may be NULL) and <psi|psi> into
  * *norm2 (host, may be NULL).  All accumulated in FP64; the energy is NOT

@@ -89,3 +89,15 @@ def test_wide_state_uses_64_bit_indices():
     n = 33
     ops = [("rx", q, 0.1 * q) for q in range(n)] + [("cz", q, q + 1) for q in range(0, n - 1, 2)]
     assert nat.precompile(n, _ops(n, ops)) >= 1
+
+
+def test_expectation_passes_of_a_wide_state():
+    """The 500-term 2-local Pauli sum on 30 qubits: read-only passes with two
+    tile buffers filled by asynchronous copies (states of 2^25+ amplitudes),
+    and the same sum on 33 qubits (64-bit indices)."""
+    from tyxonq_amd import bench_workloads as W
+    from tyxonq_amd.pauli import PauliSum
+
+    for n in (30, 33):
+        x, z, c = PauliSum(n, W.random_2local_terms(n, 120)).to_masks()
+        assert nat.ham_precompile(n, x, z, c) >= 1

@@ -123,6 +123,7 @@ _SIGS = {
     "tqsv_apply_fused": (_I, [_P, _I, _P, _I, ctypes.POINTER(PlanOpts), _P]),
     "tqsv_ham_create": (_I, [_I, _I, _P, _P, _P, ctypes.POINTER(_P)]),
     "tqsv_ham_set_coeffs": (_I, [_P, _I, _DP, _P]),
+    "tqsv_ham_precompile": (_I, [_I, _I, _P, _P, _DP, ctypes.POINTER(ctypes.c_int)]),
     "tqsv_ham_expval": (_I, [_P, _P, _DP, _DP, _DP, _P]),
     "tqsv_ham_apply": (_I, [_P, _P, _P, _P]),
     "tqsv_ham_get_info": (_I, [_P, ctypes.POINTER(HamInfo)]),
@@ -384,6 +385,20 @@ def precompile(n: int, ops: np.ndarray, *, adjoint: bool = False, tile_bits: int
     k = ctypes.c_int(0)
     _check(load().tqsv_plan_precompile(int(n), ops.ctypes.data if len(ops) else None, len(ops),
                                        ctypes.byref(o), 1 if adjoint else 0, ctypes.byref(k)))
+    return int(k.value)
+
+
+def ham_precompile(n: int, xmask, zmask, coeff) -> int:
+    """Plan a Pauli sum's expectation passes on the host and compile the
+    generated kernels Ham(n, ...) would use into the compiler's disk cache,
+    without a GPU.  Returns the number of distinct kernels."""
+    x = np.ascontiguousarray(xmask, dtype=np.uint64)
+    z = np.ascontiguousarray(zmask, dtype=np.uint64)
+    c = np.ascontiguousarray(coeff, dtype=np.float64)
+    k = ctypes.c_int(0)
+    _check(load().tqsv_ham_precompile(int(n), len(c), x.ctypes.data if len(c) else None,
+                                      z.ctypes.data if len(c) else None,
+                                      c.ctypes.data_as(_DP) if len(c) else None, ctypes.byref(k)))
     return int(k.value)
 
 

@@ -849,6 +849,50 @@ int tqsv_ham_create(int n, int nterms, const uint64_t* xmask, const uint64_t* zm
   }
 }
 
+int tqsv_ham_precompile(int n, int nterms, const uint64_t* xmask, const uint64_t* zmask,
+                        const double* coeff, int* kernels) {
+  try {
+    if (n < 1 || n > 34) return fail(TQSV_EINVAL, "tqsv_ham_precompile: n must be in [1, 34]");
+    if (nterms < 0 || (nterms > 0 && (!xmask || !zmask || !coeff)))
+      return fail(TQSV_EINVAL, "tqsv_ham_precompile: bad term arrays");
+    const uint64_t full = lowmask(n);
+    tqsv_ham h;  // host planning only: no device buffer is allocated
+    h.n = n;
+    h.nterms = nterms;
+    for (int t = 0; t < nterms; ++t) {
+      if ((xmask[t] | zmask[t]) & ~full)
+        return fail(TQSV_EINVAL, "tqsv_ham_precompile: term " + std::to_string(t) + " acts outside n qubits");
+      h.x.push_back(xmask[t]);
+      h.z.push_back(zmask[t]);
+      h.coeff.push_back(coeff[t]);
+    }
+    // the plan tqsv_ham_create makes (same policy and fallback)
+    h.jit = jit_policy(0, n, 0, 32);
+    int rc = plan_expval(&h);
+    if (rc) return rc;
+    if (h.jit && !jit_policy(0, n, h.passes.size(), 32)) {
+      h.jit = 0;
+      h.passes.clear();
+      h.prog.clear();
+      h.small_terms.clear();
+      h.big_terms.clear();
+      h.nout = 0;
+      h.partials_cap = 0;
+      rc = plan_expval(&h);
+      if (rc) return rc;
+    }
+    int k = 0;
+    if (h.jit) {
+      rc = jit_compile_only(h.passes, h.prog, n, &k);
+      if (rc) return rc;
+    }
+    if (kernels) *kernels = k;
+    return 0;
+  } catch (const std::bad_alloc&) {
+    return fail(TQSV_ENOMEM, "tqsv_ham_precompile: out of host memory");
+  }
+}
+
 int tqsv_ham_set_coeffs(tqsv_ham* h, int nterms, const double* coeff, void* stream) {
   try {
     if (!h || (nterms > 0 && !coeff)) return fail(TQSV_EINVAL, "tqsv_ham_set_coeffs: NULL argument");

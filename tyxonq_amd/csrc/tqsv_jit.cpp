@@ -124,6 +124,10 @@ std::string gen_source(int tb, const uint32_t* pw, int n) {
   // by VALU issue or cache-resident, and the extra loads cost them 3-7%.
   // TQSV_PREFETCH=1 / 0 forces the touches on / off for every pass.
   const char* pfe = std::getenv("TQSV_PREFETCH");
+  // (Two tile buffers with the next tile copied asynchronously into the
+  // other one were measured slower: 34.3 against 29.3 ms for the 500-term
+  // 30-qubit expectation.  They allow one work-group per CU, and 64 KB in
+  // flight per CU does not cover the HBM latency under load.)
   const bool prefetch = pfe && (pfe[0] == '0' || pfe[0] == '1') ? pfe[0] == '1'
                                                                  : (mode == MODE_RED && n >= 25);
   std::ostringstream o;
@@ -165,6 +169,8 @@ std::string gen_source(int tb, const uint32_t* pw, int n) {
       << "    if (tile + gridDim.x < ntiles)\n"
       << "      jit_touch_tile<" << tb << ", MODE, IDX, TM>(psi, lam, TM::base(tile + gridDim.x), tid, pf);\n";
   o << "    __syncthreads();\n";
+  // read-only passes write nothing to LDS in their phases: no barriers
+  // between them
   size_t at = PH_SIZE;
   for (uint32_t ip = 0; ip < nphase; ++ip) {
     const uint32_t* ph = pw + at;
@@ -257,7 +263,7 @@ std::string gen_source(int tb, const uint32_t* pw, int n) {
       ws += nw;
     }
     if (write) o << "      jit_store_slots<MODE" << um << ">(a, b, tA, tB, G);\n";
-    o << "      __syncthreads();\n    }\n";
+    o << (write ? "      __syncthreads();\n    }\n" : "    }\n");
     at = ws;
   }
   if (write)
@@ -381,6 +387,16 @@ int jit_compile_only(const std::vector<PassPlan>& passes, const std::vector<uint
       if (!pp.tile) continue;
       std::string src = gen_source(pp.tb, prog.data() + pp.prog_off, n);
       if (seen.emplace(src, 1).second) srcs.push_back(std::move(src));
+    }
+  }
+  // TQSV_JIT_DUMP=<directory>: also write each source there (for inspection)
+  if (const char* dir = std::getenv("TQSV_JIT_DUMP"); dir && dir[0]) {
+    for (size_t j = 0; j < srcs.size(); ++j) {
+      const std::string path = std::string(dir) + "/tqsv_pass_" + std::to_string(j) + ".hip";
+      if (FILE* f = std::fopen(path.c_str(), "w")) {
+        std::fputs(srcs[j].c_str(), f);
+        std::fclose(f);
+      }
     }
   }
   std::vector<std::string> logs(srcs.size());
