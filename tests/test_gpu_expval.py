@@ -105,6 +105,9 @@ def test_ham_apply_shared_chains_and_new_coefficients(gpu, n):
     terms = (_shared_chain_terms(rng, n, 4 if big else 12)
              + random_pauli_terms(rng, n, 4 if big else 15, maxw=3))
     terms += [(0.3, [("Z", 1)]), (-0.7, [("Z", 0), ("Z", n - 1)])]
+    # enough diagonal terms for the per-index diagonal vector (rebuilt by
+    # set_coeffs)
+    terms += [(float(rng.normal()), [("Z", q), ("Z", (q + 3) % n)]) for q in range(10)]
     h = _ham(n, terms)
     out = torch.empty(1 << n, dtype=torch.complex64, device="cuda")
     for rnd in range(2):

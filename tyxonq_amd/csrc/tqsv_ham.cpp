@@ -65,6 +65,12 @@ struct tqsv_ham {
   uint64_t* d_rtz = nullptr;
   float2* d_rtc = nullptr;
   std::vector<int> rest_order;
+  // diagonal terms summed per index (2^n floats) for H|psi>, when there are
+  // enough of them and the state is small enough (see plan_apply)
+  std::vector<int> diag_terms;
+  float* d_diag = nullptr;
+  uint64_t* d_dz = nullptr;
+  double* d_dc = nullptr;
   bool jit_ready = false;        // generated expectation kernels compiled
 
   ~tqsv_ham() {
@@ -83,6 +89,9 @@ struct tqsv_ham {
     (void)hipFree(d_rgstart);
     (void)hipFree(d_rtz);
     (void)hipFree(d_rtc);
+    (void)hipFree(d_diag);
+    (void)hipFree(d_dz);
+    (void)hipFree(d_dc);
   }
   tqsv::HamApply apply_args() const {
     tqsv::HamApply a;
@@ -94,6 +103,7 @@ struct tqsv_ham {
     a.gstart = d_rgstart;
     a.tz = d_rtz;
     a.tc = d_rtc;
+    a.diag = d_diag;
     return a;
   }
 };
@@ -382,6 +392,16 @@ std::vector<float2> sub_tables(const tqsv_ham* h) {
   return tab;
 }
 
+// the diagonal vector from the current coefficients (on stream st, waited for)
+int build_diag(tqsv_ham* h, hipStream_t st) {
+  std::vector<double> dc;
+  for (int t : h->diag_terms) dc.push_back(h->coeff[t]);
+  TQSV_HIP(hipMemcpyAsync(h->d_dc, dc.data(), dc.size() * sizeof(double), hipMemcpyHostToDevice, st));
+  TQSV_HIP(k_diag_build(h->d_diag, h->n, (uint32_t)dc.size(), h->d_dz, h->d_dc, st));
+  TQSV_HIP(hipStreamSynchronize(st));
+  return 0;
+}
+
 int plan_apply(tqsv_ham* h) {
   std::map<uint64_t, std::vector<int>> groups;
   for (int t = 0; t < h->nterms; ++t) groups[h->x[t]].push_back(t);
@@ -427,6 +447,19 @@ int plan_apply(tqsv_ham* h) {
     for (int t : kv.second) in_table[t] = 1;
     off += 1u << popc(x);
   }
+  // diagonal terms (x = 0): summed per index into a vector once (and after
+  // new coefficients) when there are at least 8 of them and the vector (4 B
+  // per amplitude) stays at most 256 MB; H|psi> then reads one float per
+  // amplitude instead of forming a parity per term
+  constexpr int kDiagMinTerms = 8;
+  constexpr int kDiagMaxQubits = 26;
+  h->diag_terms.clear();
+  if (h->n <= kDiagMaxQubits) {
+    for (int t = 0; t < h->nterms; ++t)
+      if (!h->x[t]) h->diag_terms.push_back(t);
+    if ((int)h->diag_terms.size() < kDiagMinTerms) h->diag_terms.clear();
+  }
+  for (int t : h->diag_terms) in_table[t] = 1;
   std::map<uint64_t, std::vector<int>> rest;
   for (int t = 0; t < h->nterms; ++t)
     if (!in_table[t]) rest[h->x[t]].push_back(t);
@@ -449,6 +482,15 @@ int plan_apply(tqsv_ham* h) {
       upload(&h->d_rgstart, rgstart) || upload(&h->d_rtz, rtz) ||
       upload(&h->d_rtc, apply_coeffs(h, h->rest_order)))
     return TQSV_EDEVICE;
+  if (!h->diag_terms.empty()) {
+    std::vector<uint64_t> dz;
+    for (int t : h->diag_terms) dz.push_back(h->z[t]);
+    if (upload(&h->d_dz, dz)) return TQSV_EDEVICE;
+    TQSV_HIP(hipMalloc(reinterpret_cast<void**>(&h->d_dc), h->diag_terms.size() * sizeof(double)));
+    TQSV_HIP(hipMalloc(reinterpret_cast<void**>(&h->d_diag), sizeof(float) << h->n));
+    const int rc = build_diag(h, nullptr);
+    if (rc) return rc;
+  }
   return 0;
 }
 
@@ -909,6 +951,7 @@ int tqsv_ham_set_coeffs(tqsv_ham* h, int nterms, const double* coeff, void* stre
     if (!tab.empty())
       TQSV_HIP(hipMemcpyAsync(h->d_stab, tab.data(), tab.size() * sizeof(float2), hipMemcpyHostToDevice, st));
     TQSV_HIP(hipStreamSynchronize(st));
+    if (!h->diag_terms.empty()) return build_diag(h, st);
     return 0;
   } catch (const std::bad_alloc&) {
     return fail(TQSV_ENOMEM, "tqsv_ham_set_coeffs: out of host memory");

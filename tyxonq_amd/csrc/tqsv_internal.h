@@ -94,7 +94,12 @@ struct HamApply {
   const uint32_t* gstart = nullptr;
   const uint64_t* tz = nullptr;
   const float2* tc = nullptr;
+  // the diagonal (x = 0) terms summed per index (2^n floats), when the term
+  // part leaves them out
+  const float* diag = nullptr;
 };
+hipError_t k_diag_build(float* diag, int n, uint32_t nt, const uint64_t* z, const double* c,
+                        hipStream_t st);
 // epart (optional): per block {Re, Im} of sum_i conj(psi_i) (H psi)_i, FP64
 // from the unrounded (H psi)_i; ham_apply_grid(n) blocks.
 hipError_t k_ham_apply(const float2* psi, float2* out, int n, const HamApply& H, hipStream_t st,

@@ -277,7 +277,7 @@ __global__ void ham_apply_kernel(const float2* __restrict__ psi, float2* __restr
                                  const uint64_t* __restrict__ gx,
                                  const uint32_t* __restrict__ gstart,
                                  const uint64_t* __restrict__ tz, const float2* __restrict__ tc,
-                                 double* __restrict__ epart) {
+                                 const float* __restrict__ diag, double* __restrict__ epart) {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   double er = 0.0, ei = 0.0;  // <psi|H psi> of this thread's amplitudes (with epart)
   // Work-groups are dispatched round-robin over the 8 XCDs (block id mod 8).
@@ -334,6 +334,13 @@ __global__ void ham_apply_kernel(const float2* __restrict__ psi, float2* __restr
         ar += (double)__uint_as_float(__float_as_uint(re) ^ neg);
         ai += (double)__uint_as_float(__float_as_uint(im) ^ neg);
       }
+    }
+    if (diag) {
+      // the diagonal terms (x = 0), summed per index beforehand
+      const float d = diag[i0];
+      const float2 v = psi[i];
+      ar += (double)(d * v.x);
+      ai += (double)(d * v.y);
     }
     for (uint32_t g = 0; g < ngroups; ++g) {
       const IDX j = i ^ (IDX)gx[g];
@@ -534,6 +541,32 @@ hipError_t k_pauli_dot_terms(const float2* u, const float2* v, int n, uint32_t n
   return hipGetLastError();
 }
 
+// diag[i] = sum_t c_t (-1)^{|i & z_t|}: the diagonal (x = 0) terms of a Pauli
+// sum per index, FP64 sums rounded once
+template <typename IDX>
+__global__ void diag_build_kernel(float* __restrict__ diag, uint64_t dim, uint32_t nt,
+                                  const uint64_t* __restrict__ z, const double* __restrict__ c) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i0 < dim; i0 += stride) {
+    const IDX i = (IDX)i0;
+    double s = 0.0;
+    for (uint32_t t = 0; t < nt; ++t) s += par((IDX)(i & (IDX)z[t])) ? -c[t] : c[t];
+    diag[i0] = (float)s;
+  }
+}
+
+hipError_t k_diag_build(float* diag, int n, uint32_t nt, const uint64_t* z, const double* c,
+                        hipStream_t st) {
+  const uint64_t dim = 1ull << n;
+  if (n <= 32)
+    hipLaunchKernelGGL(diag_build_kernel<uint32_t>, dim3(grid_for(dim)), dim3(BLK), 0, st, diag,
+                       dim, nt, z, c);
+  else
+    hipLaunchKernelGGL(diag_build_kernel<uint64_t>, dim3(grid_for(dim)), dim3(BLK), 0, st, diag,
+                       dim, nt, z, c);
+  return hipGetLastError();
+}
+
 uint32_t ham_apply_grid(int n) { return grid_for(1ull << n); }
 
 hipError_t k_ham_apply(const float2* psi, float2* out, int n, const HamApply& H, hipStream_t st,
@@ -541,10 +574,12 @@ hipError_t k_ham_apply(const float2* psi, float2* out, int n, const HamApply& H,
   const uint64_t dim = 1ull << n;
   if (n <= 32)
     hipLaunchKernelGGL(ham_apply_kernel<uint32_t>, dim3(grid_for(dim)), dim3(BLK), 0, st, psi, out,
-                       dim, H.nsub, H.sub, H.tab, H.ngroups, H.gx, H.gstart, H.tz, H.tc, epart);
+                       dim, H.nsub, H.sub, H.tab, H.ngroups, H.gx, H.gstart, H.tz, H.tc, H.diag,
+                       epart);
   else
     hipLaunchKernelGGL(ham_apply_kernel<uint64_t>, dim3(grid_for(dim)), dim3(BLK), 0, st, psi, out,
-                       dim, 0u, nullptr, nullptr, H.ngroups, H.gx, H.gstart, H.tz, H.tc, epart);
+                       dim, 0u, nullptr, nullptr, H.ngroups, H.gx, H.gstart, H.tz, H.tc, H.diag,
+                       epart);
   return hipGetLastError();
 }
 
