@@ -6,9 +6,8 @@ brute-force fermionic matrices in tests/test_chem.py).  The gradient with
 respect to the shared excitation parameters flows from the adjoint sweep
 through torch autograd; the oracle gradient is a central difference of the
 complex128 energy (h = 1e-4, FP64).  Tolerances: |dE| <= 1e-6 and
-|dgrad| <= 1e-5 * max(1, |grad|): the synthetic integrals give gradients up
-to about 4, and complex64 rounding over the ~900 four-flip-bit rotations of
-the 12-qubit case leaves about 3e-6 relative error in them.
+|dgrad| <= 1e-5 absolute, for gradients of up to about 4 from the synthetic
+integrals.
 """
 from __future__ import annotations
 

@@ -1089,6 +1089,54 @@ __device__ __forceinline__ void jit_load_tile(const float2* __restrict__ psi,
   }
 }
 
+// Register-staged tile copies of the pipelined gate passes.  A thread moves
+// the same 16-byte chunks as jit_load_tile / jit_store_tile (NSLOT / 2 of
+// them), but through registers: the next tile's chunks are loaded into
+// registers while the current tile is worked on in LDS, and the finished tile
+// is read out of LDS into registers before the next one is written there, so
+// that neither the loads nor the stores are waited for next to their issue.
+constexpr int kTileChunks = NSLOT / 2;
+
+template <int TB, typename IDX, typename TM, bool NT>
+__device__ __forceinline__ void jit_fetch_tile(const float2* __restrict__ psi, IDX base,
+                                               uint32_t tid, float4 (&r)[kTileChunks]) {
+  const IDX gthr = base | TM::pos(2u * tid);
+#pragma unroll
+  for (int k = 0; k < kTileChunks; ++k) {
+    const IDX g = gthr | TM::pos((uint32_t)k << (TB + 1));
+    r[k] = gload4<NT>(reinterpret_cast<const float4*>(psi) + (g >> 1));
+  }
+}
+
+template <int TB, typename IDX, typename TM, bool NT>
+__device__ __forceinline__ void jit_put_global(float2* __restrict__ psi, IDX base, uint32_t tid,
+                                               const float4 (&r)[kTileChunks]) {
+  const IDX gthr = base | TM::pos(2u * tid);
+#pragma unroll
+  for (int k = 0; k < kTileChunks; ++k) {
+    const IDX g = gthr | TM::pos((uint32_t)k << (TB + 1));
+    gstore4<NT>(reinterpret_cast<float4*>(psi) + (g >> 1), r[k]);
+  }
+}
+
+template <int TB>
+__device__ __forceinline__ void jit_put_tile(float2* tA, uint32_t tid,
+                                             const float4 (&r)[kTileChunks]) {
+  const uint32_t lthr = swz(2u * tid);
+#pragma unroll
+  for (int k = 0; k < kTileChunks; ++k)
+    *reinterpret_cast<float4*>(tA + (lthr ^ swz((uint32_t)k << (TB + 1)))) = r[k];
+}
+
+template <int TB>
+__device__ __forceinline__ void jit_get_tile(const float2* tA, uint32_t tid,
+                                             float4 (&r)[kTileChunks]) {
+  const uint32_t lthr = swz(2u * tid);
+#pragma unroll
+  for (int k = 0; k < kTileChunks; ++k)
+    r[k] = *reinterpret_cast<const float4*>(tA + (lthr ^ swz((uint32_t)k << (TB + 1))));
+}
+
 // One 4-byte load per 128-byte line (16 amplitudes) of a tile, two lines per
 // thread: issued for a work-group's next tile while it works on the current
 // one, so that the next tile's loads find their lines in the caches.  The

@@ -130,12 +130,19 @@ std::string gen_source(int tb, const uint32_t* pw, int n) {
   // flight per CU does not cover the HBM latency under load.)
   const bool prefetch = pfe && (pfe[0] == '0' || pfe[0] == '1') ? pfe[0] == '1'
                                                                  : (mode == MODE_RED && n >= 25);
+  // Pipelined gate passes: the next tile is loaded into registers while the
+  // current one is worked on in LDS, and the finished tile leaves through
+  // registers (jit_fetch_tile / jit_put_tile / jit_get_tile / jit_put_global).
+  // Gate passes without reductions only.  TQSV_PIPE=0 / 1 forces it off / on.
+  const char* ppe = std::getenv("TQSV_PIPE");
+  const bool pipe_ok = mode == MODE_GATE && write && nred == 0;
+  const bool pipe = pipe_ok && (ppe && (ppe[0] == '0' || ppe[0] == '1') ? ppe[0] == '1' : true);
   std::ostringstream o;
   // read-only passes: at least 2 waves per SIMD (at most 256 registers), as
   // the LDS of two work-groups per CU allows
   o << "#include \"tqsv_device.h\"\nusing namespace tqsv;\n"
     << "extern \"C\" __global__ void __launch_bounds__(" << NT << ") "
-    << (mode == MODE_RED ? "__attribute__((amdgpu_waves_per_eu(2))) " : "") << kernel_name(mode)
+    << (mode == MODE_RED || pipe ? "__attribute__((amdgpu_waves_per_eu(2))) " : "") << kernel_name(mode)
     << "(float2* __restrict__ psi, float2* __restrict__ lam, const uint32_t* __restrict__ prog_g,"
        " double* __restrict__ partials, uint32_t ntiles) {\n"
     << "  typedef " << (n > 32 ? "uint64_t" : "uint32_t") << " IDX;\n"
@@ -159,16 +166,29 @@ std::string gen_source(int tb, const uint32_t* pw, int n) {
   o << "  const cw_t prog = (cw_t)prog_g;\n"
     << "  const uint32_t tid = threadIdx.x;\n";
   if (nred && !racc) o << "  jit_zero_acc<" << NWAVE << ", " << nred << ">(acc, tid, " << NT << ");\n";
+  const std::string fetch = "jit_fetch_tile<" + std::to_string(tb) + ", IDX, TM, " +
+                            (nt_ld ? "true" : "false") + ">";
+  if (pipe) {
+    // the block's first tile goes to LDS, its second to registers
+    o << "  if (blockIdx.x >= ntiles) return;\n"
+      << "  float4 nx[kTileChunks];\n"
+      << "  " << fetch << "(psi, TM::base(blockIdx.x), tid, nx);\n"
+      << "  jit_put_tile<" << tb << ">(tA, tid, nx);\n"
+      << "  if (blockIdx.x + gridDim.x < ntiles) " << fetch
+      << "(psi, TM::base(blockIdx.x + gridDim.x), tid, nx);\n"
+      << "  __syncthreads();\n";
+  }
   o << "  for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {\n"
-    << "    const IDX base = TM::base(tile);\n"
-    << "    __syncthreads();\n"
-    << "    jit_load_tile<" << tb << ", MODE, IDX, TM, " << (nt_ld ? "true" : "false")
-    << ">(psi, lam, tA, tB, base, tid);\n";
+    << "    const IDX base = TM::base(tile);\n";
+  if (!pipe)
+    o << "    __syncthreads();\n"
+      << "    jit_load_tile<" << tb << ", MODE, IDX, TM, " << (nt_ld ? "true" : "false")
+      << ">(psi, lam, tA, tB, base, tid);\n";
   if (prefetch)
     o << "    float pf[4] = {0.f, 0.f, 0.f, 0.f};\n"
       << "    if (tile + gridDim.x < ntiles)\n"
       << "      jit_touch_tile<" << tb << ", MODE, IDX, TM>(psi, lam, TM::base(tile + gridDim.x), tid, pf);\n";
-  o << "    __syncthreads();\n";
+  if (!pipe) o << "    __syncthreads();\n";
   // read-only passes write nothing to LDS in their phases: no barriers
   // between them
   size_t at = PH_SIZE;
@@ -266,9 +286,23 @@ std::string gen_source(int tb, const uint32_t* pw, int n) {
     o << (write ? "      __syncthreads();\n    }\n" : "    }\n");
     at = ws;
   }
-  if (write)
+  if (pipe) {
+    // finished tile out of LDS into registers; then (all reads done) the
+    // next tile into LDS, the finished one to memory, the one after into
+    // registers.  `more` is the same for the whole block.
+    o << "    float4 out[kTileChunks];\n"
+      << "    jit_get_tile<" << tb << ">(tA, tid, out);\n"
+      << "    const bool more = tile + gridDim.x < ntiles;\n"
+      << "    if (more) {\n      __syncthreads();\n      jit_put_tile<" << tb << ">(tA, tid, nx);\n    }\n"
+      << "    jit_put_global<" << tb << ", IDX, TM, " << (nt_st ? "true" : "false")
+      << ">(psi, base, tid, out);\n"
+      << "    if (tile + 2u * gridDim.x < ntiles) " << fetch
+      << "(psi, TM::base(tile + 2u * gridDim.x), tid, nx);\n"
+      << "    if (more) __syncthreads();\n";
+  } else if (write) {
     o << "    jit_store_tile<" << tb << ", MODE, IDX, TM, " << (nt_st ? "true" : "false")
       << ">(psi, lam, tA, tB, base, tid);\n";
+  }
   if (prefetch) o << "    __asm__ volatile(\"\" :: \"v\"(pf[0]), \"v\"(pf[1]), \"v\"(pf[2]), \"v\"(pf[3]));\n";
   o << "  }\n";
   if (racc)
