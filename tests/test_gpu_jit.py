@@ -105,14 +105,19 @@ def test_update_across_rotation_forms(gpu):
         np.testing.assert_allclose(to_np(psi), want, atol=2e-5)
 
 
-def test_adjoint_generated(gpu, monkeypatch):
+@pytest.mark.parametrize("pipe,n", [(None, 12), ("1", 13), ("0", 14)])
+def test_adjoint_generated(gpu, monkeypatch, pipe, n):
+    # TQSV_PIPE=1: dual passes also stage the next psi and lambda tiles in
+    # registers; 0: no pass does (each size gets its own cached plan)
     import torch
 
     import tyxonq_amd as tq
     from tyxonq_amd.engine import StatevectorEngine
 
     monkeypatch.setenv("TQSV_JIT", "1")
-    n, layers = 12, 3
+    if pipe is not None:
+        monkeypatch.setenv("TQSV_PIPE", pipe)
+    layers = 3
     rng = np.random.default_rng(31)
     th = rng.normal(0, 0.5, size=2 * n * layers)
     terms = random_pauli_terms(rng, n, 25, maxw=3)

@@ -43,7 +43,12 @@ def _ops(n, ops, track_grad=False):
     return L.segments[0][1]
 
 
-def test_random_gate_circuit_forward_and_reverse():
+@pytest.mark.parametrize("pipe", [None, "0", "1"])
+def test_random_gate_circuit_forward_and_reverse(monkeypatch, pipe):
+    # TQSV_PIPE: register-staged tiles in gate passes (default), nowhere (0),
+    # or in gate and dual passes (1)
+    if pipe is not None:
+        monkeypatch.setenv("TQSV_PIPE", pipe)
     n = 15
     ops = random_circuit_ops(np.random.default_rng(5), n, 70)
     k = nat.precompile(n, _ops(n, ops), adjoint=True)

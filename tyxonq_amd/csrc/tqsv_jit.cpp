@@ -128,21 +128,34 @@ std::string gen_source(int tb, const uint32_t* pw, int n) {
   // other one were measured slower: 34.3 against 29.3 ms for the 500-term
   // 30-qubit expectation.  They allow one work-group per CU, and 64 KB in
   // flight per CU does not cover the HBM latency under load.)
-  const bool prefetch = pfe && (pfe[0] == '0' || pfe[0] == '1') ? pfe[0] == '1'
-                                                                 : (mode == MODE_RED && n >= 25);
-  // Pipelined gate passes: the next tile is loaded into registers while the
-  // current one is worked on in LDS, and the finished tile leaves through
+  // Pipelined passes: the next tile is loaded into registers while the
+  // current one is worked on in LDS, and a finished tile leaves through
   // registers (jit_fetch_tile / jit_put_tile / jit_get_tile / jit_put_global).
-  // Gate passes without reductions only.  TQSV_PIPE=0 / 1 forces it off / on.
+  // On by default for gate passes without reductions (30-qubit brickwork
+  // gate part 96.3 -> 90.6 ms).  Dual passes can stage psi and lambda the
+  // same way (the staged tiles live in accumulation registers, about 110 per
+  // lane), but with two tiles per work-group at 22 qubits only one load is
+  // hidden and the copies cost more: 54.4 -> 58.9 us per dual pass, so they
+  // stage only when forced.  Read-only passes keep their FP64 accumulators in
+  // registers: with 64 more for the next tile, 22 accumulators already spill
+  // 271 registers at 2 waves per SIMD, so they keep the next-tile touches.
+  // TQSV_PIPE=0 / 1 forces it off / on (1: dual passes too).
   const char* ppe = std::getenv("TQSV_PIPE");
-  const bool pipe_ok = mode == MODE_GATE && write && nred == 0;
-  const bool pipe = pipe_ok && (ppe && (ppe[0] == '0' || ppe[0] == '1') ? ppe[0] == '1' : true);
+  const bool pipe_gate = mode == MODE_GATE && write && nred == 0;
+  const bool pipe_dual = mode == MODE_DUAL && write;
+  const bool pipe = ppe && (ppe[0] == '0' || ppe[0] == '1')
+                        ? ppe[0] == '1' && (pipe_gate || pipe_dual)
+                        : pipe_gate;
+  const bool prefetch = !pipe && (pfe && (pfe[0] == '0' || pfe[0] == '1')
+                                      ? pfe[0] == '1'
+                                      : (mode == MODE_RED && n >= 25));
   std::ostringstream o;
   // read-only passes: at least 2 waves per SIMD (at most 256 registers), as
   // the LDS of two work-groups per CU allows
   o << "#include \"tqsv_device.h\"\nusing namespace tqsv;\n"
     << "extern \"C\" __global__ void __launch_bounds__(" << NT << ") "
-    << (mode == MODE_RED || pipe ? "__attribute__((amdgpu_waves_per_eu(2))) " : "") << kernel_name(mode)
+    << (mode == MODE_RED || (pipe && mode == MODE_GATE) ? "__attribute__((amdgpu_waves_per_eu(2))) " : "")
+    << kernel_name(mode)
     << "(float2* __restrict__ psi, float2* __restrict__ lam, const uint32_t* __restrict__ prog_g,"
        " double* __restrict__ partials, uint32_t ntiles) {\n"
     << "  typedef " << (n > 32 ? "uint64_t" : "uint32_t") << " IDX;\n"
@@ -168,14 +181,27 @@ std::string gen_source(int tb, const uint32_t* pw, int n) {
   if (nred && !racc) o << "  jit_zero_acc<" << NWAVE << ", " << nred << ">(acc, tid, " << NT << ");\n";
   const std::string fetch = "jit_fetch_tile<" + std::to_string(tb) + ", IDX, TM, " +
                             (nt_ld ? "true" : "false") + ">";
+  const bool two = mode == MODE_DUAL;  // psi and lambda tiles
+  // the pipelined copies of one tile: of psi (nx, tA, out) and, in dual
+  // passes, of lambda (ny, tB, outb)
+  auto fetch_both = [&](const char* tile_expr) {
+    std::string r = fetch + "(psi, TM::base(" + tile_expr + "), tid, nx);";
+    if (two) r += " " + fetch + "(lam, TM::base(" + tile_expr + "), tid, ny);";
+    return r;
+  };
+  const std::string put_both = "jit_put_tile<" + std::to_string(tb) + ">(tA, tid, nx);" +
+                               (two ? " jit_put_tile<" + std::to_string(tb) + ">(tB, tid, ny);" : "");
   if (pipe) {
-    // the block's first tile goes to LDS, its second to registers
-    o << "  if (blockIdx.x >= ntiles) return;\n"
-      << "  float4 nx[kTileChunks];\n"
-      << "  " << fetch << "(psi, TM::base(blockIdx.x), tid, nx);\n"
-      << "  jit_put_tile<" << tb << ">(tA, tid, nx);\n"
-      << "  if (blockIdx.x + gridDim.x < ntiles) " << fetch
-      << "(psi, TM::base(blockIdx.x + gridDim.x), tid, nx);\n"
+    // the block's first tile goes to LDS, its second to registers (every
+    // block reaches the end: dual passes write their partial sums there)
+    o << "  float4 nx[kTileChunks];\n";
+    if (two) o << "  float4 ny[kTileChunks];\n";
+    o << "  if (blockIdx.x < ntiles) {\n"
+      << "    " << fetch_both("blockIdx.x") << "\n"
+      << "    " << put_both << "\n"
+      << "    if (blockIdx.x + gridDim.x < ntiles) {\n      " << fetch_both("blockIdx.x + gridDim.x")
+      << "\n    }\n"
+      << "  }\n"
       << "  __syncthreads();\n";
   }
   o << "  for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {\n"
@@ -290,14 +316,19 @@ std::string gen_source(int tb, const uint32_t* pw, int n) {
     // finished tile out of LDS into registers; then (all reads done) the
     // next tile into LDS, the finished one to memory, the one after into
     // registers.  `more` is the same for the whole block.
+    const std::string store = "jit_put_global<" + std::to_string(tb) + ", IDX, TM, " +
+                              (nt_st ? "true" : "false") + ">";
     o << "    float4 out[kTileChunks];\n"
-      << "    jit_get_tile<" << tb << ">(tA, tid, out);\n"
-      << "    const bool more = tile + gridDim.x < ntiles;\n"
-      << "    if (more) {\n      __syncthreads();\n      jit_put_tile<" << tb << ">(tA, tid, nx);\n    }\n"
-      << "    jit_put_global<" << tb << ", IDX, TM, " << (nt_st ? "true" : "false")
-      << ">(psi, base, tid, out);\n"
-      << "    if (tile + 2u * gridDim.x < ntiles) " << fetch
-      << "(psi, TM::base(tile + 2u * gridDim.x), tid, nx);\n"
+      << "    jit_get_tile<" << tb << ">(tA, tid, out);\n";
+    if (two)
+      o << "    float4 outb[kTileChunks];\n"
+        << "    jit_get_tile<" << tb << ">(tB, tid, outb);\n";
+    o << "    const bool more = tile + gridDim.x < ntiles;\n"
+      << "    if (more) {\n      __syncthreads();\n      " << put_both << "\n    }\n"
+      << "    " << store << "(psi, base, tid, out);\n";
+    if (two) o << "    " << store << "(lam, base, tid, outb);\n";
+    o << "    if (tile + 2u * gridDim.x < ntiles) {\n      " << fetch_both("tile + 2u * gridDim.x")
+      << "\n    }\n"
       << "    if (more) __syncthreads();\n";
   } else if (write) {
     o << "    jit_store_tile<" << tb << ", MODE, IDX, TM, " << (nt_st ? "true" : "false")
