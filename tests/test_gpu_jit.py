@@ -107,8 +107,9 @@ def test_update_across_rotation_forms(gpu):
 
 @pytest.mark.parametrize("pipe,n", [(None, 12), ("1", 13), ("0", 14)])
 def test_adjoint_generated(gpu, monkeypatch, pipe, n):
-    # TQSV_PIPE=1: dual passes also stage the next psi and lambda tiles in
-    # registers; 0: no pass does (each size gets its own cached plan)
+    # TQSV_PIPE unset or 1: gate passes stage the next tile in registers; 0:
+    # none does (each size gets its own cached plan).  Setting the variable
+    # after earlier compiles also covers the compiler namespace's environment.
     import torch
 
     import tyxonq_amd as tq

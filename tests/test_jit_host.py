@@ -45,8 +45,8 @@ def _ops(n, ops, track_grad=False):
 
 @pytest.mark.parametrize("pipe", [None, "0", "1"])
 def test_random_gate_circuit_forward_and_reverse(monkeypatch, pipe):
-    # TQSV_PIPE: register-staged tiles in gate passes (default), nowhere (0),
-    # or in gate and dual passes (1)
+    # TQSV_PIPE: register-staged tiles in gate passes (default, or 1) or
+    # nowhere (0)
     if pipe is not None:
         monkeypatch.setenv("TQSV_PIPE", pipe)
     n = 15
@@ -106,3 +106,23 @@ def test_expectation_passes_of_a_wide_state():
     for n in (30, 33):
         x, z, c = PauliSum(n, W.random_2local_terms(n, 120)).to_masks()
         assert nat.ham_precompile(n, x, z, c) >= 1
+
+
+def test_compiles_after_the_environment_grew(monkeypatch):
+    """The compiler runs in a linker namespace with a C library of its own.
+    Environment variables added after it was loaded move the process's
+    environment array (and free the old one); the next compile must still
+    find a valid environment there (it once read the freed array and ended
+    in a segmentation fault)."""
+    n = 12
+    assert nat.precompile(n, _ops(n, random_circuit_ops(np.random.default_rng(1), n, 30))) >= 1
+    for rnd in range(3):
+        for k in range(150):
+            monkeypatch.setenv(f"TQSV_TEST_ENV_{rnd}_{k}", "x" * 40)
+        junk = [bytearray(64 * (j % 50 + 1)) for j in range(20000)]  # reuse freed blocks
+        m = 13 + rnd
+        ops = _ops(m, random_circuit_ops(np.random.default_rng(10 + rnd), m, 30))
+        assert nat.precompile(m, ops) >= 1
+        del junk
+        for k in range(0, 150, 2):
+            monkeypatch.delenv(f"TQSV_TEST_ENV_{rnd}_{k}")

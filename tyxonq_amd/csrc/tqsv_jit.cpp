@@ -24,6 +24,7 @@
 // by the process's HIP runtime.  TQSV_HIPRTC=<path> selects another hiprtc,
 // TQSV_HIPRTC=process the one the process resolved.
 #include <dlfcn.h>
+#include <unistd.h>
 #include <hip/hip_runtime.h>
 #include <hip/hiprtc.h>
 
@@ -31,6 +32,7 @@
 #include <atomic>
 #include <cstdio>
 #include <cstdlib>
+#include <memory>
 #include <mutex>
 #include <sstream>
 #include <string>
@@ -131,21 +133,16 @@ std::string gen_source(int tb, const uint32_t* pw, int n) {
   // Pipelined passes: the next tile is loaded into registers while the
   // current one is worked on in LDS, and a finished tile leaves through
   // registers (jit_fetch_tile / jit_put_tile / jit_get_tile / jit_put_global).
-  // On by default for gate passes without reductions (30-qubit brickwork
-  // gate part 96.3 -> 90.6 ms).  Dual passes can stage psi and lambda the
-  // same way (the staged tiles live in accumulation registers, about 110 per
-  // lane), but with two tiles per work-group at 22 qubits only one load is
-  // hidden and the copies cost more: 54.4 -> 58.9 us per dual pass, so they
-  // stage only when forced.  Read-only passes keep their FP64 accumulators in
-  // registers: with 64 more for the next tile, 22 accumulators already spill
-  // 271 registers at 2 waves per SIMD, so they keep the next-tile touches.
-  // TQSV_PIPE=0 / 1 forces it off / on (1: dual passes too).
+  // Gate passes without reductions only (30-qubit brickwork gate part 96.3
+  // -> 90.6 ms).  Dual passes staged the same way (psi and lambda, about 110
+  // accumulation registers per lane) were slower: with two tiles per
+  // work-group at 22 qubits only one load is hidden and the copies cost more,
+  // 54.4 -> 58.9 us per dual pass.  Read-only passes keep their FP64
+  // accumulators in registers: with 64 more for the next tile, 22
+  // accumulators already spill 271 registers at 2 waves per SIMD, so they
+  // keep the next-tile touches.  TQSV_PIPE=0 turns it off.
   const char* ppe = std::getenv("TQSV_PIPE");
-  const bool pipe_gate = mode == MODE_GATE && write && nred == 0;
-  const bool pipe_dual = mode == MODE_DUAL && write;
-  const bool pipe = ppe && (ppe[0] == '0' || ppe[0] == '1')
-                        ? ppe[0] == '1' && (pipe_gate || pipe_dual)
-                        : pipe_gate;
+  const bool pipe = mode == MODE_GATE && write && nred == 0 && !(ppe && ppe[0] == '0');
   const bool prefetch = !pipe && (pfe && (pfe[0] == '0' || pfe[0] == '1')
                                       ? pfe[0] == '1'
                                       : (mode == MODE_RED && n >= 25));
@@ -154,8 +151,7 @@ std::string gen_source(int tb, const uint32_t* pw, int n) {
   // the LDS of two work-groups per CU allows
   o << "#include \"tqsv_device.h\"\nusing namespace tqsv;\n"
     << "extern \"C\" __global__ void __launch_bounds__(" << NT << ") "
-    << (mode == MODE_RED || (pipe && mode == MODE_GATE) ? "__attribute__((amdgpu_waves_per_eu(2))) " : "")
-    << kernel_name(mode)
+    << (mode == MODE_RED || pipe ? "__attribute__((amdgpu_waves_per_eu(2))) " : "") << kernel_name(mode)
     << "(float2* __restrict__ psi, float2* __restrict__ lam, const uint32_t* __restrict__ prog_g,"
        " double* __restrict__ partials, uint32_t ntiles) {\n"
     << "  typedef " << (n > 32 ? "uint64_t" : "uint32_t") << " IDX;\n"
@@ -181,26 +177,15 @@ std::string gen_source(int tb, const uint32_t* pw, int n) {
   if (nred && !racc) o << "  jit_zero_acc<" << NWAVE << ", " << nred << ">(acc, tid, " << NT << ");\n";
   const std::string fetch = "jit_fetch_tile<" + std::to_string(tb) + ", IDX, TM, " +
                             (nt_ld ? "true" : "false") + ">";
-  const bool two = mode == MODE_DUAL;  // psi and lambda tiles
-  // the pipelined copies of one tile: of psi (nx, tA, out) and, in dual
-  // passes, of lambda (ny, tB, outb)
-  auto fetch_both = [&](const char* tile_expr) {
-    std::string r = fetch + "(psi, TM::base(" + tile_expr + "), tid, nx);";
-    if (two) r += " " + fetch + "(lam, TM::base(" + tile_expr + "), tid, ny);";
-    return r;
-  };
-  const std::string put_both = "jit_put_tile<" + std::to_string(tb) + ">(tA, tid, nx);" +
-                               (two ? " jit_put_tile<" + std::to_string(tb) + ">(tB, tid, ny);" : "");
+  const std::string put = "jit_put_tile<" + std::to_string(tb) + ">(tA, tid, nx);";
   if (pipe) {
-    // the block's first tile goes to LDS, its second to registers (every
-    // block reaches the end: dual passes write their partial sums there)
-    o << "  float4 nx[kTileChunks];\n";
-    if (two) o << "  float4 ny[kTileChunks];\n";
-    o << "  if (blockIdx.x < ntiles) {\n"
-      << "    " << fetch_both("blockIdx.x") << "\n"
-      << "    " << put_both << "\n"
-      << "    if (blockIdx.x + gridDim.x < ntiles) {\n      " << fetch_both("blockIdx.x + gridDim.x")
-      << "\n    }\n"
+    // the block's first tile goes to LDS, its second to registers
+    o << "  float4 nx[kTileChunks];\n"
+      << "  if (blockIdx.x < ntiles) {\n"
+      << "    " << fetch << "(psi, TM::base(blockIdx.x), tid, nx);\n"
+      << "    " << put << "\n"
+      << "    if (blockIdx.x + gridDim.x < ntiles) " << fetch
+      << "(psi, TM::base(blockIdx.x + gridDim.x), tid, nx);\n"
       << "  }\n"
       << "  __syncthreads();\n";
   }
@@ -319,16 +304,12 @@ std::string gen_source(int tb, const uint32_t* pw, int n) {
     const std::string store = "jit_put_global<" + std::to_string(tb) + ", IDX, TM, " +
                               (nt_st ? "true" : "false") + ">";
     o << "    float4 out[kTileChunks];\n"
-      << "    jit_get_tile<" << tb << ">(tA, tid, out);\n";
-    if (two)
-      o << "    float4 outb[kTileChunks];\n"
-        << "    jit_get_tile<" << tb << ">(tB, tid, outb);\n";
-    o << "    const bool more = tile + gridDim.x < ntiles;\n"
-      << "    if (more) {\n      __syncthreads();\n      " << put_both << "\n    }\n"
-      << "    " << store << "(psi, base, tid, out);\n";
-    if (two) o << "    " << store << "(lam, base, tid, outb);\n";
-    o << "    if (tile + 2u * gridDim.x < ntiles) {\n      " << fetch_both("tile + 2u * gridDim.x")
-      << "\n    }\n"
+      << "    jit_get_tile<" << tb << ">(tA, tid, out);\n"
+      << "    const bool more = tile + gridDim.x < ntiles;\n"
+      << "    if (more) {\n      __syncthreads();\n      " << put << "\n    }\n"
+      << "    " << store << "(psi, base, tid, out);\n"
+      << "    if (tile + 2u * gridDim.x < ntiles) " << fetch
+      << "(psi, TM::base(tile + 2u * gridDim.x), tid, nx);\n"
       << "    if (more) __syncthreads();\n";
   } else if (write) {
     o << "    jit_store_tile<" << tb << ", MODE, IDX, TM, " << (nt_st ? "true" : "false")
@@ -355,6 +336,8 @@ struct Rtc {
   decltype(&hiprtcGetCode) get_code = nullptr;
   decltype(&hiprtcDestroyProgram) destroy = nullptr;
   std::string origin;
+  // the environment pointer of the compiler namespace's own C library
+  char*** ns_environ = nullptr;
 };
 
 template <typename F>
@@ -380,6 +363,7 @@ const Rtc& rtc() {
             bind(h, "hiprtcGetCodeSize", x.code_size) && bind(h, "hiprtcGetCode", x.get_code) &&
             bind(h, "hiprtcDestroyProgram", x.destroy)) {
           x.origin = path;
+          x.ns_environ = reinterpret_cast<char***>(dlsym(h, "__environ"));
           return x;
         }
       }
@@ -395,6 +379,36 @@ const Rtc& rtc() {
     return x;
   }();
   return r;
+}
+
+// The compiler namespace (dlmopen) has a C library of its own, whose
+// environment pointer was copied from the process's when the namespace was
+// loaded.  setenv in the process may since have moved that array and freed
+// the old one, which the compiler's getenv calls would then read (a
+// segmentation fault inside the next compile).  Before each batch of
+// compiles the namespace is pointed at a private copy of the process's
+// current environment.  Copies are kept for the life of the process (one per
+// distinct environment, a few KB each): a compile on another thread may
+// still read the previous one.
+void sync_compiler_environment(const Rtc& R) {
+  if (!R.ns_environ) return;
+  struct EnvCopy {
+    std::vector<std::string> strs;
+    std::vector<char*> ptrs;
+  };
+  static std::mutex mu;
+  static std::vector<std::unique_ptr<EnvCopy>> copies;
+  std::lock_guard<std::mutex> lk(mu);
+  std::vector<std::string> cur;
+  for (char** e = environ; e && *e; ++e) cur.emplace_back(*e);
+  if (copies.empty() || copies.back()->strs != cur) {
+    auto c = std::make_unique<EnvCopy>();
+    c->strs = std::move(cur);
+    for (std::string& v : c->strs) c->ptrs.push_back(&v[0]);
+    c->ptrs.push_back(nullptr);
+    copies.push_back(std::move(c));
+  }
+  *R.ns_environ = copies.back()->ptrs.data();
 }
 
 bool compile_source(const std::string& src, std::vector<char>& code, std::string& log) {
@@ -466,6 +480,7 @@ int jit_compile_only(const std::vector<PassPlan>& passes, const std::vector<uint
   }
   std::vector<std::string> logs(srcs.size());
   std::vector<char> ok(srcs.size(), 0);
+  sync_compiler_environment(rtc());
   std::atomic<size_t> next{0};
   auto work = [&]() {
     std::vector<char> code;
@@ -506,6 +521,7 @@ int jit_prepare(std::vector<PassPlan>& passes, const std::vector<uint32_t>& prog
     std::vector<std::vector<char>> codes(todo.size());
     std::vector<std::string> logs(todo.size());
     std::vector<char> ok(todo.size(), 0);
+    sync_compiler_environment(rtc());
     std::atomic<size_t> next{0};
     auto work = [&]() {
       for (size_t j; (j = next.fetch_add(1)) < todo.size();)
