@@ -143,6 +143,21 @@ std::string gen_source(int tb, const uint32_t* pw, int n) {
   // keep the next-tile touches.  TQSV_PIPE=0 turns it off.
   const char* ppe = std::getenv("TQSV_PIPE");
   const bool pipe = mode == MODE_GATE && write && nred == 0 && !(ppe && ppe[0] == '0');
+  // Dual passes with one tile buffer: psi and lambda go through the same
+  // 64 KB of LDS in turn (loads, phase changes, stores), so that two
+  // work-groups fit per CU instead of one.  Only when every phase loads and
+  // stores all slots: phases that skip slots rely on both tiles staying in
+  // LDS.  TQSV_DUAL1=0 / 1 forces it off / on where it applies.
+  bool all_slots = true;
+  for (uint32_t ip = 0, at = PH_SIZE; ip < nphase; ++ip) {
+    const uint32_t ns = pw[at + PS_NSTEPS];
+    if (phase_slots(pw, at + PS_SIZE, ns) != 0xffffffffu) all_slots = false;
+    at += PS_SIZE;
+    for (uint32_t k = 0; k < ns; ++k) at += pw[at] >> 16;
+  }
+  const char* d1e = std::getenv("TQSV_DUAL1");
+  const bool one_buf = mode == MODE_DUAL && write && all_slots &&
+                       !(d1e && d1e[0] == '0');
   const bool prefetch = !pipe && (pfe && (pfe[0] == '0' || pfe[0] == '1')
                                       ? pfe[0] == '1'
                                       : (mode == MODE_RED && n >= 25));
@@ -151,7 +166,8 @@ std::string gen_source(int tb, const uint32_t* pw, int n) {
   // the LDS of two work-groups per CU allows
   o << "#include \"tqsv_device.h\"\nusing namespace tqsv;\n"
     << "extern \"C\" __global__ void __launch_bounds__(" << NT << ") "
-    << (mode == MODE_RED || pipe ? "__attribute__((amdgpu_waves_per_eu(2))) " : "") << kernel_name(mode)
+    << (mode == MODE_RED || pipe || one_buf ? "__attribute__((amdgpu_waves_per_eu(2))) " : "")
+    << kernel_name(mode)
     << "(float2* __restrict__ psi, float2* __restrict__ lam, const uint32_t* __restrict__ prog_g,"
        " double* __restrict__ partials, uint32_t ntiles) {\n"
     << "  typedef " << (n > 32 ? "uint64_t" : "uint32_t") << " IDX;\n"
@@ -160,7 +176,7 @@ std::string gen_source(int tb, const uint32_t* pw, int n) {
   o << "> TM;\n"
     << "  constexpr int MODE = " << mode << ";\n"
     << "  __shared__ __align__(16) float2 tA[" << TSIZE << "];\n";
-  if (mode == MODE_DUAL)
+  if (mode == MODE_DUAL && !one_buf)
     o << "  __shared__ __align__(16) float2 tB[" << TSIZE << "];\n";
   else
     o << "  float2* tB = nullptr;\n";
@@ -191,7 +207,16 @@ std::string gen_source(int tb, const uint32_t* pw, int n) {
   }
   o << "  for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {\n"
     << "    const IDX base = TM::base(tile);\n";
-  if (!pipe)
+  const std::string ld1 = "jit_load_tile<" + std::to_string(tb) + ", MODE_GATE, IDX, TM, " +
+                          (nt_ld ? "true" : "false") + ">";
+  const std::string st1 = "jit_store_tile<" + std::to_string(tb) + ", MODE_GATE, IDX, TM, " +
+                          (nt_st ? "true" : "false") + ">";
+  if (one_buf)
+    o << "    PhaseGeo<IDX> G, Gp;\n"
+      << "    float2 a[NSLOT], b[NSLOT];\n"
+      << "    __syncthreads();\n"
+      << "    " << ld1 << "(psi, lam, tA, tB, base, tid);\n";
+  else if (!pipe)
     o << "    __syncthreads();\n"
       << "    jit_load_tile<" << tb << ", MODE, IDX, TM, " << (nt_ld ? "true" : "false")
       << ">(psi, lam, tA, tB, base, tid);\n";
@@ -206,13 +231,36 @@ std::string gen_source(int tb, const uint32_t* pw, int n) {
   for (uint32_t ip = 0; ip < nphase; ++ip) {
     const uint32_t* ph = pw + at;
     const uint32_t nsteps = ph[PS_NSTEPS];
-    o << "    {\n      PhaseGeo<IDX> G;\n      jit_geo<TM, IDX";
-    for (int j = 0; j < RB; ++j) o << ", " << ph[PS_R + j];
     const uint32_t used = phase_slots(pw, at + PS_SIZE, nsteps);
     const std::string um = used == 0xffffffffu ? "" : ", " + std::to_string(used) + "u";
-    o << ">(G, base, tid);\n"
-      << "      float2 a[NSLOT], b[NSLOT];\n"
-      << "      jit_load_slots<MODE" << um << ">(a, b, tA, tB, G);\n";
+    if (one_buf) {
+      // psi's slots move to this phase's positions through the buffer, then
+      // lambda's (at the first phase: psi's slots, then lambda's tile)
+      o << "    {\n" << (ip ? "      Gp = G;\n" : "") << "      jit_geo<TM, IDX";
+      for (int j = 0; j < RB; ++j) o << ", " << ph[PS_R + j];
+      o << ">(G, base, tid);\n";
+      if (ip)
+        o << "      __syncthreads();\n"
+          << "      jit_store_slots<MODE_GATE>(a, b, tA, tB, Gp);\n"
+          << "      __syncthreads();\n"
+          << "      jit_load_slots<MODE_GATE>(a, b, tA, tB, G);\n"
+          << "      __syncthreads();\n"
+          << "      jit_store_slots<MODE_GATE>(b, a, tA, tB, Gp);\n"
+          << "      __syncthreads();\n"
+          << "      jit_load_slots<MODE_GATE>(b, a, tA, tB, G);\n";
+      else
+        o << "      jit_load_slots<MODE_GATE>(a, b, tA, tB, G);\n"
+          << "      __syncthreads();\n"
+          << "      " << ld1 << "(lam, psi, tA, tB, base, tid);\n"
+          << "      __syncthreads();\n"
+          << "      jit_load_slots<MODE_GATE>(b, a, tA, tB, G);\n";
+    } else {
+      o << "    {\n      PhaseGeo<IDX> G;\n      jit_geo<TM, IDX";
+      for (int j = 0; j < RB; ++j) o << ", " << ph[PS_R + j];
+      o << ">(G, base, tid);\n"
+        << "      float2 a[NSLOT], b[NSLOT];\n"
+        << "      jit_load_slots<MODE" << um << ">(a, b, tA, tB, G);\n";
+    }
     size_t ws = at + PS_SIZE;
     for (uint32_t is = 0; is < nsteps; ++is) {
       const uint32_t* w = pw + ws;
@@ -293,10 +341,23 @@ std::string gen_source(int tb, const uint32_t* pw, int n) {
       }
       ws += nw;
     }
-    if (write) o << "      jit_store_slots<MODE" << um << ">(a, b, tA, tB, G);\n";
-    o << (write ? "      __syncthreads();\n    }\n" : "    }\n");
+    if (one_buf)
+      o << "    }\n";
+    else if (write)
+      o << "      jit_store_slots<MODE" << um << ">(a, b, tA, tB, G);\n      __syncthreads();\n    }\n";
+    else
+      o << "    }\n";
     at = ws;
   }
+  if (one_buf)
+    o << "    __syncthreads();\n"
+      << "    jit_store_slots<MODE_GATE>(a, b, tA, tB, G);\n"
+      << "    __syncthreads();\n"
+      << "    " << st1 << "(psi, lam, tA, tB, base, tid);\n"
+      << "    __syncthreads();\n"
+      << "    jit_store_slots<MODE_GATE>(b, a, tA, tB, G);\n"
+      << "    __syncthreads();\n"
+      << "    " << st1 << "(lam, psi, tA, tB, base, tid);\n";
   if (pipe) {
     // finished tile out of LDS into registers; then (all reads done) the
     // next tile into LDS, the finished one to memory, the one after into
@@ -311,7 +372,7 @@ std::string gen_source(int tb, const uint32_t* pw, int n) {
       << "    if (tile + 2u * gridDim.x < ntiles) " << fetch
       << "(psi, TM::base(tile + 2u * gridDim.x), tid, nx);\n"
       << "    if (more) __syncthreads();\n";
-  } else if (write) {
+  } else if (write && !one_buf) {
     o << "    jit_store_tile<" << tb << ", MODE, IDX, TM, " << (nt_st ? "true" : "false")
       << ">(psi, lam, tA, tB, base, tid);\n";
   }
