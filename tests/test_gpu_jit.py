@@ -105,19 +105,22 @@ def test_update_across_rotation_forms(gpu):
         np.testing.assert_allclose(to_np(psi), want, atol=2e-5)
 
 
-@pytest.mark.parametrize("pipe,n", [(None, 12), ("1", 13), ("0", 14)])
-def test_adjoint_generated(gpu, monkeypatch, pipe, n):
+@pytest.mark.parametrize("env,n", [({}, 12), ({"TQSV_PIPE": "1"}, 13), ({"TQSV_PIPE": "0"}, 14),
+                                   ({"TQSV_DUAL1": "0"}, 15)])
+def test_adjoint_generated(gpu, monkeypatch, env, n):
     # TQSV_PIPE unset or 1: gate passes stage the next tile in registers; 0:
-    # none does (each size gets its own cached plan).  Setting the variable
-    # after earlier compiles also covers the compiler namespace's environment.
+    # none does.  TQSV_DUAL1=0: dual passes keep psi and lambda in two tile
+    # buffers instead of one.  Each size gets its own cached plan; setting the
+    # variables after earlier compiles also covers the compiler namespace's
+    # environment.
     import torch
 
     import tyxonq_amd as tq
     from tyxonq_amd.engine import StatevectorEngine
 
     monkeypatch.setenv("TQSV_JIT", "1")
-    if pipe is not None:
-        monkeypatch.setenv("TQSV_PIPE", pipe)
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
     layers = 3
     rng = np.random.default_rng(31)
     th = rng.normal(0, 0.5, size=2 * n * layers)

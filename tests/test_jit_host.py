@@ -43,12 +43,12 @@ def _ops(n, ops, track_grad=False):
     return L.segments[0][1]
 
 
-@pytest.mark.parametrize("pipe", [None, "0", "1"])
-def test_random_gate_circuit_forward_and_reverse(monkeypatch, pipe):
-    # TQSV_PIPE: register-staged tiles in gate passes (default, or 1) or
-    # nowhere (0)
-    if pipe is not None:
-        monkeypatch.setenv("TQSV_PIPE", pipe)
+@pytest.mark.parametrize("env", [{}, {"TQSV_PIPE": "0"}, {"TQSV_DUAL1": "0"}])
+def test_random_gate_circuit_forward_and_reverse(monkeypatch, env):
+    # TQSV_PIPE=0: no register-staged tiles in gate passes; TQSV_DUAL1=0: dual
+    # passes with two tile buffers
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
     n = 15
     ops = random_circuit_ops(np.random.default_rng(5), n, 70)
     k = nat.precompile(n, _ops(n, ops), adjoint=True)
