@@ -56,7 +56,13 @@ struct tqsv_ham {
   // subgroups (n <= 32) and the remaining terms as flip groups
   std::vector<uint32_t> sub;               // 4 words per subgroup
   std::vector<std::vector<int>> sub_terms;  // terms of each subgroup
-  uint32_t* d_sub = nullptr;
+  // the same subgroups as H|psi> reads them: those alone on their flip mask
+  // (4 words each, as in sub), and blocks of the subgroups that share one
+  // (header {x, shifts, first member, end member}, members {zc, table offset})
+  std::vector<uint32_t> sub1, blk, mem;
+  uint32_t* d_sub1 = nullptr;
+  uint32_t* d_blk = nullptr;
+  uint32_t* d_mem = nullptr;
   float2* d_stab = nullptr;
   size_t stab_size = 0;
   uint32_t nrest = 0;
@@ -83,7 +89,9 @@ struct tqsv_ham {
     (void)hipFree(d_gstart);
     (void)hipFree(d_tz);
     (void)hipFree(d_tc);
-    (void)hipFree(d_sub);
+    (void)hipFree(d_sub1);
+    (void)hipFree(d_blk);
+    (void)hipFree(d_mem);
     (void)hipFree(d_stab);
     (void)hipFree(d_rgx);
     (void)hipFree(d_rgstart);
@@ -95,9 +103,13 @@ struct tqsv_ham {
   }
   tqsv::HamApply apply_args() const {
     tqsv::HamApply a;
-    a.nsub = (uint32_t)(sub.size() / 4);
-    a.sub = d_sub;
+    a.nsub = (uint32_t)(sub1.size() / 4);
+    a.sub = d_sub1;
+    a.nblk = (uint32_t)(blk.size() / 4);
+    a.blk = d_blk;
+    a.mem = d_mem;
     a.tab = d_stab;
+    a.ntab = stab_size;
     a.ngroups = nrest;
     a.gx = d_rgx;
     a.gstart = d_rgstart;
@@ -476,9 +488,29 @@ int plan_apply(tqsv_ham* h) {
   }
   rgstart.push_back((uint32_t)rtz.size());
   h->nrest = (uint32_t)rgx.size();
+  // subgroups sharing a flip mask (consecutive in sub, which is ordered by x)
+  // form a block: one partner load, pattern and complex product for all of
+  // them, one table load and Z sign per member
+  h->sub1.clear();
+  h->blk.clear();
+  h->mem.clear();
+  const size_t nsubg = h->sub.size() / 4;
+  for (size_t a = 0; a < nsubg;) {
+    size_t b = a + 1;
+    while (b < nsubg && h->sub[4 * b] == h->sub[4 * a]) ++b;
+    if (b - a == 1) {
+      h->sub1.insert(h->sub1.end(), h->sub.begin() + 4 * a, h->sub.begin() + 4 * a + 4);
+    } else {
+      const uint32_t m0 = (uint32_t)(h->mem.size() / 2);
+      for (size_t k = a; k < b; ++k) h->mem.insert(h->mem.end(), {h->sub[4 * k + 1], h->sub[4 * k + 2]});
+      h->blk.insert(h->blk.end(), {h->sub[4 * a], h->sub[4 * a + 3], m0, m0 + (uint32_t)(b - a)});
+    }
+    a = b;
+  }
   const std::vector<float2> tab = sub_tables(h);
   h->stab_size = tab.size();
-  if (upload(&h->d_sub, h->sub) || upload(&h->d_stab, tab) || upload(&h->d_rgx, rgx) ||
+  if (upload(&h->d_sub1, h->sub1) || upload(&h->d_blk, h->blk) ||
+      upload(&h->d_mem, h->mem) || upload(&h->d_stab, tab) || upload(&h->d_rgx, rgx) ||
       upload(&h->d_rgstart, rgstart) || upload(&h->d_rtz, rtz) ||
       upload(&h->d_rtc, apply_coeffs(h, h->rest_order)))
     return TQSV_EDEVICE;

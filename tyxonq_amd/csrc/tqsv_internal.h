@@ -89,6 +89,12 @@ struct HamApply {
   uint32_t nsub = 0;
   const uint32_t* sub = nullptr;
   const float2* tab = nullptr;
+  size_t ntab = 0;  // table entries
+  // blocks of subgroups that share a flip mask: header {x, shifts, first
+  // member, end member}, members {zc, table offset}
+  uint32_t nblk = 0;
+  const uint32_t* blk = nullptr;
+  const uint32_t* mem = nullptr;
   uint32_t ngroups = 0;
   const uint64_t* gx = nullptr;
   const uint32_t* gstart = nullptr;

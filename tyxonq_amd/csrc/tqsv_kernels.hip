@@ -265,15 +265,23 @@ __global__ __launch_bounds__(BLK) void pauli_dot_terms_kernel(const float2* u, c
 // (-1)^{|j & z_t|} over the group's terms; one thread per output amplitude.
 // Terms that share x and the Z mask outside x (zc) come as one table
 // subgroup: c(j) = (-1)^{|j & zc|} tab[pattern of j on the bits of x], one
-// coefficient load instead of one parity and sign per term (the 14251 terms of
-// a 20-spin-orbital Jordan-Wigner Hamiltonian are 2445 such subgroups of 4
-// plus 102 groups).  Headers and term arrays are read-only and indexed
+// coefficient load instead of one parity and sign per term.  Subgroups that
+// share a flip mask form a block with one partner load, pattern and complex
+// product (the 14251 terms of a 20-spin-orbital Jordan-Wigner Hamiltonian
+// are 2445 subgroups alone on their 4-bit flip masks, 90 blocks of 19
+// subgroups on 2-bit flip masks, and 211 diagonal terms summed into a vector
+// beforehand).  Headers and term arrays are read-only and indexed
 // uniformly across the wave, so they come through scalar loads; signs are an
 // XOR of the sign bit with a parity (32-bit indices up to 32 qubits).
-template <typename IDX>
+// OFF32 (states of at most 2^29 amplitudes): the table part gathers through
+// 32-bit byte offsets from the uniform base addresses, without 64-bit address
+// arithmetic per load.
+template <typename IDX, bool OFF32 = false>
 __global__ void ham_apply_kernel(const float2* __restrict__ psi, float2* __restrict__ out,
                                  uint64_t dim, uint32_t nsub, const uint32_t* __restrict__ sub,
-                                 const float2* __restrict__ tab, uint32_t ngroups,
+                                 const float2* __restrict__ tab, uint32_t nblk,
+                                 const uint32_t* __restrict__ blk,
+                                 const uint32_t* __restrict__ mem, uint32_t ngroups,
                                  const uint64_t* __restrict__ gx,
                                  const uint32_t* __restrict__ gstart,
                                  const uint64_t* __restrict__ tz, const float2* __restrict__ tc,
@@ -303,6 +311,12 @@ __global__ void ham_apply_kernel(const float2* __restrict__ psi, float2* __restr
         return ((j >> (pk & 31u)) & 1u) | ((j >> ((pk >> 5) & 31u)) & 2u) |
                ((j >> ((pk >> 10) & 31u)) & 4u) | ((j >> ((pk >> 15) & 31u)) & 8u);
       };
+      auto gather = [](const float2* base, uint32_t k) -> float2 {
+        if constexpr (OFF32)
+          return *reinterpret_cast<const float2*>(reinterpret_cast<const char*>(base) + (k << 3));
+        else
+          return base[k];
+      };
       uint32_t g = 0;
       for (; g + 4 <= nsub; g += 4) {
         uint32_t jj[4], zc[4];
@@ -312,27 +326,52 @@ __global__ void ham_apply_kernel(const float2* __restrict__ psi, float2* __restr
           const uint32_t* d = sub + 4 * (g + u);
           jj[u] = (uint32_t)i ^ d[0];
           zc[u] = d[1];
-          v[u] = psi[jj[u]];
-          c[u] = tab[d[2] + pattern(jj[u], d[0], d[3])];
+          v[u] = gather(psi, jj[u]);
+          c[u] = gather(tab, d[2] + pattern(jj[u], d[0], d[3]));
         }
+        // the round's four products are summed in FP32 (as exact as the
+        // products themselves), the rounds in FP64
+        float sr = 0.f, si = 0.f;
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
           const uint32_t neg = par(jj[u] & zc[u]) << 31;
           const float re = c[u].x * v[u].x - c[u].y * v[u].y;
           const float im = c[u].x * v[u].y + c[u].y * v[u].x;
-          ar += (double)__uint_as_float(__float_as_uint(re) ^ neg);
-          ai += (double)__uint_as_float(__float_as_uint(im) ^ neg);
+          sr += __uint_as_float(__float_as_uint(re) ^ neg);
+          si += __uint_as_float(__float_as_uint(im) ^ neg);
         }
+        ar += (double)sr;
+        ai += (double)si;
       }
       for (; g < nsub; ++g) {
         const uint32_t* d = sub + 4 * g;
         const uint32_t j = (uint32_t)i ^ d[0];
-        const float2 v = psi[j];
-        const float2 c = tab[d[2] + pattern(j, d[0], d[3])];
+        const float2 v = gather(psi, j);
+        const float2 c = gather(tab, d[2] + pattern(j, d[0], d[3]));
         const uint32_t neg = par(j & d[1]) << 31;
         const float re = c.x * v.x - c.y * v.y, im = c.x * v.y + c.y * v.x;
         ar += (double)__uint_as_float(__float_as_uint(re) ^ neg);
         ai += (double)__uint_as_float(__float_as_uint(im) ^ neg);
+      }
+      // subgroups sharing a flip mask: one partner load and pattern, the
+      // members' signed coefficients summed (FP32), one complex product
+      for (uint32_t bb = 0; bb < nblk; ++bb) {
+        const uint32_t* hd = blk + 4 * bb;
+        const uint32_t j = (uint32_t)i ^ hd[0];
+        const float2 v = gather(psi, j);
+        const uint32_t q = pattern(j, hd[0], hd[1]);
+        float cr = 0.f, ci = 0.f;
+        const uint32_t m1 = hd[3];
+#pragma unroll 4
+        for (uint32_t m = hd[2]; m < m1; ++m) {
+          const uint32_t* md = mem + 2 * m;
+          const float2 c = gather(tab, md[1] + q);
+          const uint32_t neg = par(j & md[0]) << 31;
+          cr += __uint_as_float(__float_as_uint(c.x) ^ neg);
+          ci += __uint_as_float(__float_as_uint(c.y) ^ neg);
+        }
+        ar += (double)(cr * v.x - ci * v.y);
+        ai += (double)(cr * v.y + ci * v.x);
       }
     }
     if (diag) {
@@ -572,14 +611,18 @@ uint32_t ham_apply_grid(int n) { return grid_for(1ull << n); }
 hipError_t k_ham_apply(const float2* psi, float2* out, int n, const HamApply& H, hipStream_t st,
                        double* epart) {
   const uint64_t dim = 1ull << n;
-  if (n <= 32)
+  if (n <= 29 && H.ntab < (1ull << 29))
+    hipLaunchKernelGGL((ham_apply_kernel<uint32_t, true>), dim3(grid_for(dim)), dim3(BLK), 0, st, psi,
+                       out, dim, H.nsub, H.sub, H.tab, H.nblk, H.blk, H.mem, H.ngroups, H.gx,
+                       H.gstart, H.tz, H.tc, H.diag, epart);
+  else if (n <= 32)
     hipLaunchKernelGGL(ham_apply_kernel<uint32_t>, dim3(grid_for(dim)), dim3(BLK), 0, st, psi, out,
-                       dim, H.nsub, H.sub, H.tab, H.ngroups, H.gx, H.gstart, H.tz, H.tc, H.diag,
-                       epart);
+                       dim, H.nsub, H.sub, H.tab, H.nblk, H.blk, H.mem, H.ngroups, H.gx, H.gstart,
+                       H.tz, H.tc, H.diag, epart);
   else
     hipLaunchKernelGGL(ham_apply_kernel<uint64_t>, dim3(grid_for(dim)), dim3(BLK), 0, st, psi, out,
-                       dim, 0u, nullptr, nullptr, H.ngroups, H.gx, H.gstart, H.tz, H.tc, H.diag,
-                       epart);
+                       dim, 0u, nullptr, nullptr, 0u, nullptr, nullptr, H.ngroups, H.gx, H.gstart,
+                       H.tz, H.tc, H.diag, epart);
   return hipGetLastError();
 }
 
