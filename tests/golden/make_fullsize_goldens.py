@@ -4,7 +4,7 @@ spec runs out of letters for n >= 23, libs/quantum_library/kernels/
 statevector.py:50-55, and a 30-qubit complex128 state does not fit twice in
 host memory).  Run once in the build container (CPU only, hours at 30 qubits):
 
-    python tests/golden/make_fullsize_goldens.py [--only brickwork30,adjoint22,ucc20]
+    python tests/golden/make_fullsize_goldens.py [--only brickwork30,adjoint22,adjoint26,ucc20]
 
 Computed with the complex128 C restatement (oracle/sv128.c through
 oracle/c128.py, pinned by tests/test_oracle_c128.py), gate by gate in the
@@ -16,6 +16,8 @@ reference's order:
                ~ N(0, 0.5^2) (seed 1234), periodic Heisenberg chain + 0.5 sum Z:
                E and the full gradient by the reverse sweep, plus a sample of
                24 components by the parameter-shift rule (common.py:11-23).
+  adjoint26    the same ansatz at 26 qubits, 2 layers, 104 parameters ~ N(0, 0.5^2)
+               (default_rng(7)): E and the full gradient by the reverse sweep.
   ucc20        20 spin-orbitals, 10 electrons, synthetic integrals (seed 1234),
                UCCSD with spin-shared parameters, theta ~ N(0, 0.05^2) from
                default_rng(1234): E and the full gradient by the reverse sweep,
@@ -101,6 +103,22 @@ def adjoint22() -> dict:
             "ps_grad": [float(v) for v in ps], "cpu_s": round(time.time() - t0, 1)}
 
 
+def adjoint26() -> dict:
+    # two layers at 26 qubits: 8192 tiles, so a dual pass runs several tiles
+    # per work-group; parameters from default_rng(7), as the test draws them
+    n, layers = 26, 2
+    th = np.random.default_rng(7).normal(0.0, 0.5, size=2 * n * layers)
+    c, _ = W.ryrz_ring_ansatz(n, layers, params=list(th))
+    terms = W.heisenberg_ring_terms(n, 0.5)
+    t0 = time.time()
+    gates = C.gates_of(c.ops, n, param_of=[(k, 1.0) for k in range(th.size)])
+    e, grad, nrm = C.energy_and_grad(n, gates, terms, th.size)
+    log(f"adjoint26 E={e:.12f} |g|={np.linalg.norm(grad):.6f}")
+    return {"qubits": n, "layers": layers, "seed": 7, "params": int(th.size),
+            "hamiltonian": "periodic Heisenberg XX+YY+ZZ + 0.5 sum Z", "energy": float(e),
+            "norm2": nrm, "grad": [float(v) for v in grad], "cpu_s": round(time.time() - t0, 1)}
+
+
 def ucc_setup(ns: int = 10, seed: int = SEED):
     no = ns // 2
     n = 2 * ns
@@ -160,7 +178,8 @@ def main() -> None:
                       "tests/golden/make_fullsize_goldens.py")
     for name in args.only.split(","):
         log(f"start {name}")
-        data[name] = {"adjoint22": adjoint22, "ucc20": ucc20, "brickwork30": brickwork30}[name]()
+        data[name] = {"adjoint22": adjoint22, "adjoint26": adjoint26, "ucc20": ucc20,
+                      "brickwork30": brickwork30}[name]()
         OUT.write_text(json.dumps(data, indent=1) + "\n")
         log(f"wrote {name} to {OUT}")
 
