@@ -120,6 +120,11 @@ def test_ham_apply_shared_chains_and_new_coefficients(gpu, n):
 
 
 def test_ham_apply_jordan_wigner_hamiltonian(gpu):
+    """A Jordan-Wigner Hamiltonian: hopping terms times number operators share
+    a 2-bit flip mask with Z masks one bit apart (the chunk-table blocks of
+    the apply kernel), plus terms on one flip mask whose Z masks differ in
+    two bits (the member loop).  Against the oracle, again after new
+    coefficients."""
     import torch
 
     from tyxonq_amd import chem
@@ -128,12 +133,21 @@ def test_ham_apply_jordan_wigner_hamiltonian(gpu):
     n = 2 * ns
     hh, gg = chem.synthetic_integrals(ns, 7)
     terms = chem.molecular_hamiltonian(hh, gg)
-    v = rand_state(np.random.default_rng(3), n)
+    # one flip mask (qubits 1, 6), Z masks {2, 3} x {4, 8}: two bits apart
+    for zs in ([2, 4], [3, 4], [2, 8], [3, 8], [2, 3, 4, 8]):
+        terms.append((0.05 * len(zs), [("X", 1), ("Y", 6)] + [("Z", q) for q in zs]))
+    rng = np.random.default_rng(3)
+    v = rand_state(rng, n)
     h = _ham(n, terms)
     out = torch.empty(1 << n, dtype=torch.complex64, device="cuda")
     h.apply(to_gpu(v), out)
     want = sum(c * O.pauli_apply(v, fac, n) for c, fac in terms)
     np.testing.assert_allclose(to_np(out), want, atol=2e-5)
+    c2 = rng.normal(size=len(terms))
+    h.set_coeffs(c2)
+    h.apply(to_gpu(v), out)
+    want = sum(c * O.pauli_apply(v, fac, n) for c, (_, fac) in zip(c2, terms))
+    np.testing.assert_allclose(to_np(out), want, atol=5e-5)
 
 
 def test_expval_deterministic(gpu):

@@ -63,6 +63,19 @@ struct tqsv_ham {
   uint32_t* d_sub1 = nullptr;
   uint32_t* d_blk = nullptr;
   uint32_t* d_mem = nullptr;
+  // blocks whose members' Z masks differ from one base mask z0 in at most one
+  // bit r each: sum_m T_m[q] (-1)^{|j & zc_m|} = (-1)^{|j & z0|} sum over
+  // chunks of tables indexed by q and 4 consecutive bits of j (header {x,
+  // shifts, z0, first chunk, end chunk}, chunks {lowest bit, table offset});
+  // cdefs: per chunk the members (subgroup, bit - lowest bit, -1 for the base)
+  std::vector<uint32_t> cblk, chk;
+  struct ChunkDef {
+    int xbits = 0;
+    std::vector<std::pair<int, int>> members;
+  };
+  std::vector<ChunkDef> cdefs;
+  uint32_t* d_cblk = nullptr;
+  uint32_t* d_chk = nullptr;
   float2* d_stab = nullptr;
   size_t stab_size = 0;
   uint32_t nrest = 0;
@@ -92,6 +105,8 @@ struct tqsv_ham {
     (void)hipFree(d_sub1);
     (void)hipFree(d_blk);
     (void)hipFree(d_mem);
+    (void)hipFree(d_cblk);
+    (void)hipFree(d_chk);
     (void)hipFree(d_stab);
     (void)hipFree(d_rgx);
     (void)hipFree(d_rgstart);
@@ -108,6 +123,9 @@ struct tqsv_ham {
     a.nblk = (uint32_t)(blk.size() / 4);
     a.blk = d_blk;
     a.mem = d_mem;
+    a.ncblk = (uint32_t)(cblk.size() / 5);
+    a.cblk = d_cblk;
+    a.chk = d_chk;
     a.tab = d_stab;
     a.ntab = stab_size;
     a.ngroups = nrest;
@@ -383,8 +401,10 @@ std::vector<float2> apply_coeffs(const tqsv_ham* h, const std::vector<int>& orde
 // subgroup with flip mask x = sum_t c_t i^{k_t} (-1)^{|dep_x(q) & z_t|}
 std::vector<float2> sub_tables(const tqsv_ham* h) {
   static const std::complex<double> ipow[4] = {{1, 0}, {0, 1}, {-1, 0}, {0, -1}};
-  std::vector<float2> tab;
+  std::vector<std::complex<double>> t64;  // every subgroup's table, FP64
+  std::vector<size_t> at(h->sub_terms.size());
   for (size_t g = 0; g < h->sub_terms.size(); ++g) {
+    at[g] = t64.size();
     const uint32_t x = h->sub[4 * g];
     int pos[32];
     int m = 0;
@@ -398,8 +418,25 @@ std::vector<float2> sub_tables(const tqsv_ham* h) {
         const double sg = (popc(dep & h->z[t]) & 1) ? -1.0 : 1.0;
         c += sg * h->coeff[t] * ipow[popc(h->x[t] & h->z[t]) & 3];
       }
-      tab.push_back(make_float2((float)c.real(), (float)c.imag()));
+      t64.push_back(c);
     }
+  }
+  std::vector<float2> tab;
+  tab.reserve(t64.size());
+  for (const auto& c : t64) tab.push_back(make_float2((float)c.real(), (float)c.imag()));
+  // chunk tables: entry (q, v) = sum over the chunk's members of T_m[q], times
+  // (-1)^(bit of v at the member's Z bit) (the base member: no sign)
+  for (const auto& cd : h->cdefs) {
+    const uint32_t nq = 1u << cd.xbits;
+    for (uint32_t q = 0; q < nq; ++q)
+      for (uint32_t v = 0; v < 16; ++v) {
+        std::complex<double> c(0, 0);
+        for (const auto& mb : cd.members) {
+          const double sg = mb.second >= 0 && ((v >> mb.second) & 1u) ? -1.0 : 1.0;
+          c += sg * t64[at[(size_t)mb.first] + q];
+        }
+        tab.push_back(make_float2((float)c.real(), (float)c.imag()));
+      }
   }
   return tab;
 }
@@ -494,11 +531,59 @@ int plan_apply(tqsv_ham* h) {
   h->sub1.clear();
   h->blk.clear();
   h->mem.clear();
+  h->cblk.clear();
+  h->chk.clear();
+  h->cdefs.clear();
   const size_t nsubg = h->sub.size() / 4;
+  uint32_t coff = off;  // chunk tables follow the subgroup tables
   for (size_t a = 0; a < nsubg;) {
     size_t b = a + 1;
     while (b < nsubg && h->sub[4 * b] == h->sub[4 * a]) ++b;
-    if (b - a == 1) {
+    // chunk form: a base Z mask z0 among the members from which every
+    // member differs in at most one bit
+    int base = -1;
+    for (size_t k0 = a; b - a >= 3 && k0 < b && base < 0; ++k0) {
+      bool ok = true;
+      for (size_t k = a; k < b && ok; ++k) ok = popc(h->sub[4 * k + 1] ^ h->sub[4 * k0 + 1]) <= 1;
+      if (ok) base = (int)k0;
+    }
+    std::vector<int> rbits;
+    if (base >= 0) {
+      const uint32_t z0 = h->sub[4 * (size_t)base + 1];
+      for (size_t k = a; k < b; ++k)
+        if (const uint32_t d = h->sub[4 * k + 1] ^ z0) rbits.push_back(ctz(d));
+      std::sort(rbits.begin(), rbits.end());
+      size_t nch = 0;
+      for (size_t r = 0; r < rbits.size(); ++nch) {
+        const int lo = rbits[r];
+        while (r < rbits.size() && rbits[r] < lo + 4) ++r;
+      }
+      if (nch == 0 || nch >= b - a) base = -1;  // no gain over the member loop
+    }
+    if (base >= 0) {
+      const uint32_t x = h->sub[4 * a], z0 = h->sub[4 * (size_t)base + 1];
+      const int xbits = popc(x);
+      const uint32_t c0 = (uint32_t)(h->chk.size() / 2);
+      size_t r = 0;
+      while (r < rbits.size()) {
+        const int lo = rbits[r];
+        tqsv_ham::ChunkDef cd;
+        cd.xbits = xbits;
+        for (size_t k = a; k < b; ++k) {
+          const uint32_t d = h->sub[4 * k + 1] ^ z0;
+          if (d == 0) {
+            if (c0 == h->chk.size() / 2) cd.members.push_back({(int)k, -1});  // base: first chunk
+          } else if (ctz(d) >= lo && ctz(d) < lo + 4) {
+            cd.members.push_back({(int)k, ctz(d) - lo});
+          }
+        }
+        while (r < rbits.size() && rbits[r] < lo + 4) ++r;
+        h->chk.insert(h->chk.end(), {(uint32_t)lo, coff});
+        coff += (1u << xbits) * 16u;
+        h->cdefs.push_back(std::move(cd));
+      }
+      h->cblk.insert(h->cblk.end(), {x, h->sub[4 * a + 3], z0, c0, (uint32_t)(h->chk.size() / 2)});
+    } else if (b - a == 1) {
       h->sub1.insert(h->sub1.end(), h->sub.begin() + 4 * a, h->sub.begin() + 4 * a + 4);
     } else {
       const uint32_t m0 = (uint32_t)(h->mem.size() / 2);
@@ -510,7 +595,8 @@ int plan_apply(tqsv_ham* h) {
   const std::vector<float2> tab = sub_tables(h);
   h->stab_size = tab.size();
   if (upload(&h->d_sub1, h->sub1) || upload(&h->d_blk, h->blk) ||
-      upload(&h->d_mem, h->mem) || upload(&h->d_stab, tab) || upload(&h->d_rgx, rgx) ||
+      upload(&h->d_mem, h->mem) || upload(&h->d_cblk, h->cblk) || upload(&h->d_chk, h->chk) ||
+      upload(&h->d_stab, tab) || upload(&h->d_rgx, rgx) ||
       upload(&h->d_rgstart, rgstart) || upload(&h->d_rtz, rtz) ||
       upload(&h->d_rtc, apply_coeffs(h, h->rest_order)))
     return TQSV_EDEVICE;

@@ -95,6 +95,11 @@ struct HamApply {
   uint32_t nblk = 0;
   const uint32_t* blk = nullptr;
   const uint32_t* mem = nullptr;
+  // blocks in chunk form: header {x, shifts, z0, first chunk, end chunk},
+  // chunks {lowest bit, table offset} (tables of 2^|x| x 16 entries)
+  uint32_t ncblk = 0;
+  const uint32_t* cblk = nullptr;
+  const uint32_t* chk = nullptr;
   uint32_t ngroups = 0;
   const uint64_t* gx = nullptr;
   const uint32_t* gstart = nullptr;

@@ -281,7 +281,9 @@ __global__ void ham_apply_kernel(const float2* __restrict__ psi, float2* __restr
                                  uint64_t dim, uint32_t nsub, const uint32_t* __restrict__ sub,
                                  const float2* __restrict__ tab, uint32_t nblk,
                                  const uint32_t* __restrict__ blk,
-                                 const uint32_t* __restrict__ mem, uint32_t ngroups,
+                                 const uint32_t* __restrict__ mem, uint32_t ncblk,
+                                 const uint32_t* __restrict__ cblk,
+                                 const uint32_t* __restrict__ chk, uint32_t ngroups,
                                  const uint64_t* __restrict__ gx,
                                  const uint32_t* __restrict__ gstart,
                                  const uint64_t* __restrict__ tz, const float2* __restrict__ tc,
@@ -372,6 +374,28 @@ __global__ void ham_apply_kernel(const float2* __restrict__ psi, float2* __restr
         }
         ar += (double)(cr * v.x - ci * v.y);
         ai += (double)(cr * v.y + ci * v.x);
+      }
+      // blocks in chunk form: the members differ from the base Z mask z0 in
+      // one bit each, so their signed sum is (-1)^{|j & z0|} times a sum of
+      // chunk tables, each indexed by the pattern and 4 consecutive bits of j
+      for (uint32_t bb = 0; bb < ncblk; ++bb) {
+        const uint32_t* hd = cblk + 5 * bb;
+        const uint32_t j = (uint32_t)i ^ hd[0];
+        const float2 v = gather(psi, j);
+        const uint32_t qs = pattern(j, hd[0], hd[1]) << 4;
+        float cr = 0.f, ci = 0.f;
+        const uint32_t c1 = hd[4];
+#pragma unroll 4
+        for (uint32_t cc = hd[3]; cc < c1; ++cc) {
+          const uint32_t* cd = chk + 2 * cc;
+          const float2 c = gather(tab, cd[1] + qs + ((j >> cd[0]) & 15u));
+          cr += c.x;
+          ci += c.y;
+        }
+        const uint32_t neg = par(j & hd[2]) << 31;
+        const float re = cr * v.x - ci * v.y, im = cr * v.y + ci * v.x;
+        ar += (double)__uint_as_float(__float_as_uint(re) ^ neg);
+        ai += (double)__uint_as_float(__float_as_uint(im) ^ neg);
       }
     }
     if (diag) {
@@ -613,16 +637,16 @@ hipError_t k_ham_apply(const float2* psi, float2* out, int n, const HamApply& H,
   const uint64_t dim = 1ull << n;
   if (n <= 29 && H.ntab < (1ull << 29))
     hipLaunchKernelGGL((ham_apply_kernel<uint32_t, true>), dim3(grid_for(dim)), dim3(BLK), 0, st, psi,
-                       out, dim, H.nsub, H.sub, H.tab, H.nblk, H.blk, H.mem, H.ngroups, H.gx,
-                       H.gstart, H.tz, H.tc, H.diag, epart);
+                       out, dim, H.nsub, H.sub, H.tab, H.nblk, H.blk, H.mem, H.ncblk, H.cblk,
+                       H.chk, H.ngroups, H.gx, H.gstart, H.tz, H.tc, H.diag, epart);
   else if (n <= 32)
     hipLaunchKernelGGL(ham_apply_kernel<uint32_t>, dim3(grid_for(dim)), dim3(BLK), 0, st, psi, out,
-                       dim, H.nsub, H.sub, H.tab, H.nblk, H.blk, H.mem, H.ngroups, H.gx, H.gstart,
-                       H.tz, H.tc, H.diag, epart);
+                       dim, H.nsub, H.sub, H.tab, H.nblk, H.blk, H.mem, H.ncblk, H.cblk, H.chk,
+                       H.ngroups, H.gx, H.gstart, H.tz, H.tc, H.diag, epart);
   else
     hipLaunchKernelGGL(ham_apply_kernel<uint64_t>, dim3(grid_for(dim)), dim3(BLK), 0, st, psi, out,
-                       dim, 0u, nullptr, nullptr, 0u, nullptr, nullptr, H.ngroups, H.gx, H.gstart,
-                       H.tz, H.tc, H.diag, epart);
+                       dim, 0u, nullptr, nullptr, 0u, nullptr, nullptr, 0u, nullptr, nullptr,
+                       H.ngroups, H.gx, H.gstart, H.tz, H.tc, H.diag, epart);
   return hipGetLastError();
 }
 
