@@ -817,7 +817,7 @@ void emit_phase(std::vector<Gate>& gates, const std::vector<int>& sel,
           Gate gen = g;
           for (int r = 0; r < g.nmem; ++r) gen.ma[r] = g.mg[r];
           PgrpClass cls[kMaxGroup];
-          const int nc = pgrp_classes(gen, cls);
+          const int nc = pgrp_classes(gen, cls, false);  // active flags only
           int xb[64];
           int m = 0;
           for (uint64_t v = g.x; v; v &= v - 1) xb[m++] = ctz(v);
@@ -1451,9 +1451,10 @@ bool same_structure(const std::vector<Gate>& a, const std::vector<Gate>& b) {
     for (int r = 0; r < x.nmem; ++r)
       if (x.mz[r] != y.mz[r] || (x.mp[r] >= 0) != (y.mp[r] >= 0)) return false;
     if (x.kind == GK_PGRP) {
+      // only the active flags are compared: no class constants
       PgrpClass cx[kMaxGroup], cy[kMaxGroup];
-      const int nc = pgrp_classes(x, cx);
-      pgrp_classes(y, cy);
+      const int nc = pgrp_classes(x, cx, false);
+      pgrp_classes(y, cy, false);
       for (int c = 0; c < nc; ++c)
         if (cx[c].active != cy[c].active) return false;
     }
