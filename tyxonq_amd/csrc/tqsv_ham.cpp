@@ -696,6 +696,31 @@ static size_t count_tile(const std::vector<PassPlan>& passes) {
   return k;
 }
 
+// Generated kernels for a freshly emitted program.  The source of a pass
+// depends only on its non-value words, so when the program with its value
+// words zeroed (the skeleton) equals the one last compiled for this plan, the
+// previous build's kernels are kept and no source is generated; otherwise
+// they are compiled or taken from the process-wide cache (jit_prepare).
+static int attach_kernels(std::vector<PassPlan>& passes, const std::vector<uint32_t>& prog,
+                          const std::vector<uint32_t>& fpos, const std::vector<PassPlan>& old,
+                          std::vector<uint32_t>& skel, std::vector<uint32_t>& work, int n) {
+  work.assign(prog.begin(), prog.end());
+  for (uint32_t i : fpos) work[i] = 0;
+  bool same = !skel.empty() && work == skel && old.size() == passes.size();
+  for (size_t i = 0; same && i < passes.size(); ++i)
+    same = passes[i].tile == old[i].tile && passes[i].tb == old[i].tb &&
+           passes[i].prog_off == old[i].prog_off && (!passes[i].tile || old[i].jit_fn);
+  if (same) {
+    for (size_t i = 0; i < passes.size(); ++i) passes[i].jit_fn = old[i].jit_fn;
+    return 0;
+  }
+  skel.clear();
+  const int rc = jit_prepare(passes, prog, n);
+  if (rc) return rc;
+  skel.swap(work);
+  return 0;
+}
+
 int plan_build_forward(tqsv_plan* p, const tqsv_op* ops, int nops, hipStream_t st) {
   std::vector<Gate>& g = p->gwork;
   int rc = gates_from_ops(p->nu, ops, nops, g);
@@ -716,15 +741,19 @@ int plan_build_forward(tqsv_plan* p, const tqsv_op* ops, int nops, hipStream_t s
   passes.clear();
   prog.clear();
   uint32_t red_total = 0;
-  rc = emit_passes(g, p->so, sel, passes, prog, red_total);
+  std::vector<uint32_t>& fpos = p->fposwork;
+  fpos.clear();
+  rc = emit_passes(g, p->so, sel, passes, prog, red_total, &fpos);
   if (rc) return rc;
   p->fsel.swap(sel);
   p->max_param = -1;
   for (const Gate& x : g) p->max_param = std::max(p->max_param, x.param);
   p->fgen = jit_policy(p->jit_req, p->n, count_tile(passes), 64) != 0;
   if (p->fgen) {
-    rc = jit_prepare(passes, prog, p->n);
+    rc = attach_kernels(passes, prog, fpos, p->passes, p->fskel, p->skelwork, p->n);
     if (rc) return rc;
+  } else {
+    p->fskel.clear();
   }
   p->gates.swap(g);
   p->passes.swap(passes);
@@ -832,13 +861,17 @@ int plan_build_adjoint(tqsv_plan* p, hipStream_t st) {
   passes.clear();
   prog.clear();
   uint32_t red_total = 0;
-  rc = emit_passes(r, so, sel, passes, prog, red_total);
+  std::vector<uint32_t>& fpos = p->fposwork;
+  fpos.clear();
+  rc = emit_passes(r, so, sel, passes, prog, red_total, &fpos);
   if (rc) return rc;
   p->rsel.swap(sel);
   p->rgen = jit_policy(p->jit_req, p->n, count_tile(passes), 64) != 0;
   if (p->rgen) {
-    rc = jit_prepare(passes, prog, p->n);
+    rc = attach_kernels(passes, prog, fpos, p->rpasses, p->rskel, p->skelwork, p->n);
     if (rc) return rc;
+  } else {
+    p->rskel.clear();
   }
   p->rgates.swap(r);
   p->rpasses.swap(passes);

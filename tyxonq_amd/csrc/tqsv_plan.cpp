@@ -202,11 +202,13 @@ bool fusable(const Gate& g) {
 
 struct Emit {
   std::vector<uint32_t>& w;
-  explicit Emit(std::vector<uint32_t>& v) : w(v) {}
+  std::vector<uint32_t>* fpos;  // positions of value words, or null
+  explicit Emit(std::vector<uint32_t>& v, std::vector<uint32_t>* fp = nullptr) : w(v), fpos(fp) {}
   void u(uint32_t v) { w.push_back(v); }
   void f(float x) {
     uint32_t v;
     std::memcpy(&v, &x, 4);
+    if (fpos) fpos->push_back((uint32_t)w.size());
     w.push_back(v);
   }
   void c(cd v) {
@@ -1352,7 +1354,8 @@ int select_passes(std::vector<Gate>& gates, const SchedOpts& so, std::vector<Pas
 
 
 int emit_passes(std::vector<Gate>& gates, const SchedOpts& so, std::vector<PassPlan> plan,
-                std::vector<PassPlan>& passes, std::vector<uint32_t>& prog, uint32_t& red_total) {
+                std::vector<PassPlan>& passes, std::vector<uint32_t>& prog, uint32_t& red_total,
+                std::vector<uint32_t>* fpos) {
   const int n = so.n;
   const int T = effective_tile_bits(n, so.tile_bits, so.dual);
   const int TBv = T - RB;
@@ -1383,7 +1386,7 @@ int emit_passes(std::vector<Gate>& gates, const SchedOpts& so, std::vector<PassP
     for (uint32_t k = 0; k < pp.m; ++k) geo.H.push_back((int)pp.H[k]);
     pp.prog_off = prog.size();
     pp.red_base = red_total;
-    Emit e(prog);
+    Emit e(prog, fpos);
     for (int k = 0; k < PH_SIZE; ++k) e.u(0);
     uint32_t nred = 0;
     for (size_t ip = 0; ip < pp.phases.size(); ++ip) {

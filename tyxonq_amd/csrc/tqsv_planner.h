@@ -148,8 +148,12 @@ int schedule(std::vector<Gate>& gates, const SchedOpts& so, std::vector<PassPlan
 // whose gates keep their structure (same_structure) when only parameters
 // change reuses its selection and re-emits.
 int select_passes(std::vector<Gate>& gates, const SchedOpts& so, std::vector<PassPlan>& plan);
+// fpos (optional) receives the positions in prog of the value words (angles,
+// phases, scales: everything written as a float); the other words fix the
+// program's structure and the generated source of each pass.
 int emit_passes(std::vector<Gate>& gates, const SchedOpts& so, std::vector<PassPlan> plan,
-                std::vector<PassPlan>& passes, std::vector<uint32_t>& prog, uint32_t& red_total);
+                std::vector<PassPlan>& passes, std::vector<uint32_t>& prog, uint32_t& red_total,
+                std::vector<uint32_t>* fpos = nullptr);
 bool same_structure(const std::vector<Gate>& a, const std::vector<Gate>& b);
 
 // Run planned passes (device program at dprog; partial-sum scratch for reductions).
@@ -229,6 +233,9 @@ struct tqsv_plan {
   std::vector<tqsv::Gate> gwork, rwork;
   std::vector<tqsv::PassPlan> pwork, rpwork;
   std::vector<uint32_t> progwork, rprogwork;
+  // programs with their value words zeroed, as last compiled (generated
+  // kernels): an update whose skeleton is unchanged keeps the kernels
+  std::vector<uint32_t> fskel, rskel, skelwork, fposwork;
   // States of fewer than kPadQubits qubits run as kPadQubits-qubit states whose
   // extra high index bits stay zero (no gate touches them), so that they use
   // the tile passes instead of one launch per gate: the caller's 2^nu
