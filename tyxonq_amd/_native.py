@@ -575,6 +575,20 @@ class Sampler:
         self.max_shots = int(max_shots)
         self._h = ctypes.c_void_p()
         _check(load().tqsv_sampler_create(self.n, self.max_shots, ctypes.byref(self._h)))
+        self._pin = None  # pinned (index, count) staging, allocated on first use
+
+    def _staging(self, cap: int):
+        """Page-locked host buffers for the (index, count) results, so that the
+        device-to-host copies run as direct DMA (allocated once per sampler,
+        sized for its largest result)."""
+        if self._pin is None or self._pin[0].shape[0] < cap:
+            import torch
+
+            size = max(cap, min(self.max_shots, 1 << self.n))
+            ti = torch.empty(size, dtype=torch.int64, pin_memory=True)
+            tc = torch.empty(size, dtype=torch.int64, pin_memory=True)
+            self._pin = (ti.numpy().view(np.uint64), tc.numpy(), ti, tc)
+        return self._pin[0], self._pin[1]
 
     def sample(self, psi, shots: int, seed: int = 0):
         """(index, count) arrays of the distinct sampled basis states, index
@@ -583,13 +597,13 @@ class Sampler:
             raise ValueError(f"sampler is for {self.n} qubits, state has {_nqubits(psi)}")
         shots = int(shots)
         cap = max(1, min(shots, 1 << self.n))
-        idx = np.empty(cap, dtype=np.uint64)
-        cnt = np.empty(cap, dtype=np.int64)
+        idx, cnt = self._staging(cap)
         nu = ctypes.c_int64(0)
         _check(load().tqsv_sampler_run(self._h, _ptr(psi), shots, int(seed) & ((1 << 64) - 1),
                                        idx.ctypes.data, cnt.ctypes.data, ctypes.byref(nu),
                                        stream_ptr()))
         k = int(nu.value)
+        # the staging is reused by the next call: the results are copied out
         return idx[:k].copy(), cnt[:k].copy()
 
     def close(self) -> None:
